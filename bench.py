@@ -1,0 +1,300 @@
+"""Benchmark of the refined mixed-precision LU solve against the plain FP64 LU.
+
+Workload (one "step" = one system solved from scratch, inputs resident in
+HBM): A = U diag(s) V^T of order n = 16384 with U, V orthogonal (QR of
+Gaussian matrices), s log-spaced on [1, kappa = 1e4], and nrhs = 1 right-hand
+side b ~ N(0, 1).  A mixed step factors a scaled copy of A in low precision
+and refines in FP64 to a normwise backward error <= 1e-13; a plain step
+copies A and runs the FP64 PartialPivLu + solve (the drop-in path).  Every
+step's backward error is checked against 1e-13.
+
+Prints one JSON line: value = mixed solves/s over the timed steps, with
+p50/p99 step latency, the plain path's numbers beside it, the roofline of
+the dominant kernel (from HIP events around every launch in the timed
+region), and the single-core CPU baseline (the oracle's restatement of
+rulinalg's PartialPivLu::decompose + solve, timed on a bounded sample and
+scaled by n^3).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/s + p99 latency at FP64 backward err ≤1e-13: mixed-refined vs plain LU"
+PEAKS = {"f64": 78.6, "f32": 157.3, "f16": 2500.0, "bf16": 2500.0}  # dense MFMA TFLOP/s
+HBM_PEAK = 8000.0  # GB/s
+PREC = {"f32": 1, "f16": 2, "bf16": 3}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n", type=int, default=16384)
+    p.add_argument("--nrhs", type=int, default=1)
+    p.add_argument("--kappa", type=float, default=1e4)
+    p.add_argument("--prec", default="f16", choices=sorted(PREC))
+    p.add_argument("--plain-steps", type=int, default=None)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-extras", action="store_true")
+    p.add_argument("--cpu-n", type=int, default=2048)
+    return p.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU layout is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from rulinalg_amd import _lib
+    lib = _lib.load()
+    assert lib.rlg_set_device(0) == 0, _lib.last_error()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    sptr = C.c_void_p(stream.cuda_stream)
+    n, nrhs = args.n, args.nrhs
+
+    # ------------------------------------------------------------ inputs
+    log(f"generating A = U diag(s) V^T, n={n}, kappa={args.kappa:g}")
+    t0 = time.time()
+    g = torch.Generator(device=dev)
+
+    def orthogonal(seed):
+        g.manual_seed(seed)
+        q, r = torch.linalg.qr(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+        q *= torch.sign(torch.diagonal(r)).unsqueeze(0)
+        return q
+
+    s = torch.logspace(0.0, float(np.log10(args.kappa)), n, dtype=torch.float64, device=dev)
+    us = orthogonal(20) * s.unsqueeze(0)
+    vt = orthogonal(21).t().contiguous()
+    A = torch.empty(n, n, dtype=torch.float64, device=dev)
+    assert lib.rlg_gemm_f64_dev(n, n, n, 1.0, us.data_ptr(), n, vt.data_ptr(), n, 0.0,
+                                A.data_ptr(), n, sptr) == 0, _lib.last_error()
+    del us, vt
+    g.manual_seed(22)
+    B = torch.randn(nrhs, n, dtype=torch.float64, device=dev, generator=g)
+    X = torch.empty_like(B)
+    work = torch.empty_like(A)
+    torch.cuda.synchronize()
+    log(f"inputs ready in {time.time() - t0:.1f}s")
+
+    opts = _lib.MixedOpts(PREC[args.prec], 0, 1e-13, 30, 1)
+    rep = _lib.MixedReport()
+    eta = C.c_double()
+
+    def mixed_step():
+        st = lib.rlg_solve_mixed_f64_dev(n, A.data_ptr(), n, nrhs, B.data_ptr(), n, X.data_ptr(), n,
+                                         C.byref(opts), C.byref(rep), sptr)
+        assert st == 0, _lib.last_error()
+        return rep.backward_error, rep.fell_back, rep.iterations
+
+    def plain_step():
+        work.copy_(A)
+        h = C.c_void_p()
+        st = lib.rlg_lu_decompose_f64_dev(n, work.data_ptr(), n, sptr, C.byref(h))
+        assert st == 0, _lib.last_error()
+        st = lib.rlg_lu_solve_f64_dev(h, nrhs, B.data_ptr(), n, X.data_ptr(), n, sptr)
+        assert st == 0, _lib.last_error()
+        lib.rlg_lu_free(h)
+
+    def timed(fn, steps, check):
+        lat, etas, extra = [], [], []
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for _ in range(steps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            out = fn()
+            e1.record(stream)
+            e1.synchronize()
+            lat.append(e0.elapsed_time(e1))
+            extra.append(out)
+            if check:
+                etas.append(check())
+        torch.cuda.synchronize()
+        return time.perf_counter() - t_start, lat, etas, extra
+
+    def check_eta():
+        st = lib.rlg_backward_error_f64_dev(n, A.data_ptr(), n, nrhs, B.data_ptr(), n, X.data_ptr(),
+                                            n, C.byref(eta), sptr)
+        assert st == 0, _lib.last_error()
+        return eta.value
+
+    # ------------------------------------------------------------ mixed
+    log(f"warmup {args.warmup} mixed step(s), prec={args.prec}")
+    for _ in range(args.warmup):
+        mixed_step()
+    lib.rlg_prof_reset()
+    lib.rlg_prof_enable(1)
+    log(f"timing {args.steps} mixed steps")
+    wall, lat, etas, extra = timed(mixed_step, args.steps, check_eta)
+    lib.rlg_prof_enable(0)
+    prof = {}
+    buf = C.create_string_buffer(1 << 16)
+    lib.rlg_prof_dump(buf, len(buf))
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms, wk = line.split()
+        prof[name] = (int(cnt), float(ms), float(wk))
+    mixed_ms = float(np.sum(lat))
+    value = args.steps / (mixed_ms / 1e3)
+    eta_max = max(etas)
+    assert eta_max <= 1e-13, f"mixed backward error {eta_max:.3e} > 1e-13"
+    fell_back = sum(e[1] for e in extra)
+    iters = [e[2] for e in extra]
+    log(f"mixed: {np.mean(lat):.2f} ms/step, eta max {eta_max:.2e}, IR iters {iters}, fallbacks {fell_back}")
+
+    # ------------------------------------------------------------ plain
+    psteps = args.plain_steps if args.plain_steps is not None else max(1, args.steps // 2)
+    log(f"plain FP64 LU: warmup 1, timing {psteps}")
+    plain_step()
+    pwall, plat, petas, _ = timed(plain_step, psteps, check_eta)
+    assert max(petas) <= 1e-13, f"plain backward error {max(petas):.3e}"
+    plain_value = psteps / (np.sum(plat) / 1e3)
+    log(f"plain: {np.mean(plat):.2f} ms/step, eta max {max(petas):.2e}")
+
+    # ------------------------------------------------------------ roofline
+    # dominant kernel class of the mixed path in the timed region
+    gemm_classes = {"gemm_f64": "f64", "gemm_f32": "f32", "gemm_f16": "f16", "gemm_bf16": "bf16"}
+    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    name, (cnt, ms, wk) = dom
+    if name in gemm_classes and wk > 0:
+        achieved = wk / (ms / 1e3) / 1e12
+        peak = PEAKS[gemm_classes[name]]
+        roof = {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                "launches": cnt, "avg_launch_ms": round(ms / cnt, 5),
+                "flops_per_launch": wk / cnt}
+    else:
+        roof = {"kernel": name, "bound": "hbm", "achieved": None, "peak": HBM_PEAK, "unit": "GB/s",
+                "frac": None, "traffic": None, "launches": cnt, "avg_launch_ms": round(ms / cnt, 5)}
+    breakdown = {k: {"launches": v[0], "ms_per_step": round(v[1] / args.steps, 3)} for k, v in
+                 sorted(prof.items(), key=lambda kv: -kv[1][1])}
+
+    result = {
+        "metric": METRIC, "value": round(value, 4), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(mixed_ms / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": f"{args.prec} factor + f32 panels, f64 refinement", "data": "synthetic",
+        "config": {"workload": f"refined mixed-precision LU solve, A = U diag(s) V^T, n={n}, "
+                               f"kappa={args.kappa:g}, {nrhs} RHS", "n": n, "nrhs": nrhs,
+                   "kappa": args.kappa, "factor_prec": args.prec, "tol": 1e-13},
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "backward_error_max": eta_max, "ir_iterations": iters, "fallbacks": fell_back,
+        "plain": {"value": round(plain_value, 4), "unit": "solves/s",
+                  "p50_ms": round(float(np.percentile(plat, 50)), 3),
+                  "p99_ms": round(float(np.percentile(plat, 99)), 3),
+                  "backward_error_max": max(petas), "steps": psteps},
+        "speedup_vs_plain": round(value / plain_value, 3),
+        "roofline": roof, "kernel_breakdown": breakdown,
+    }
+
+    # ------------------------------------------------------------ extras
+    if not args.no_extras:
+        del work
+        torch.cuda.empty_cache()
+        result.update(extras(lib, torch, dev, sptr, stream))
+
+    # ------------------------------------------------------------ CPU baseline
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(args.cpu_n, n)
+    print(json.dumps(result), flush=True)
+
+
+def extras(lib, torch, dev, sptr, stream):
+    """DGEMM 8192^3 and Cholesky 16384 (plain FP64) timings."""
+    from rulinalg_amd import _lib
+    out = {}
+    g = torch.Generator(device=dev)
+    m = 8192
+    g.manual_seed(10)
+    a = torch.rand(m, m, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    g.manual_seed(11)
+    b = torch.rand(m, m, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    c = torch.empty_like(a)
+    lib.rlg_gemm_f64_dev(m, m, m, 1.0, a.data_ptr(), m, b.data_ptr(), m, 0.0, c.data_ptr(), m, sptr)
+    times = []
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        assert lib.rlg_gemm_f64_dev(m, m, m, 1.0, a.data_ptr(), m, b.data_ptr(), m, 0.0,
+                                    c.data_ptr(), m, sptr) == 0
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    # Freivalds probe against a compensated host product of a few columns
+    z = torch.randn(m, 4, dtype=torch.float64, device=dev)
+    err = (c @ z - a @ (b @ z)).abs().max().item() / (a.abs().sum(1).max().item() * b.abs().max().item())
+    tf = 2 * m ** 3 / (min(times) / 1e3) / 1e12
+    out["dgemm_8192"] = {"ms": round(min(times), 3), "tflops": round(tf, 2),
+                         "frac_fp64_peak": round(tf / 78.6, 4), "freivalds_rel": err}
+    log(f"DGEMM 8192: {min(times):.2f} ms, {tf:.1f} TF/s")
+    del a, b, c
+    n = 16384
+    g.manual_seed(14)
+    G = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+    S = torch.empty_like(G)
+    Gt = G.t().contiguous()
+    lib.rlg_gemm_f64_dev(n, n, n, 1.0 / n, G.data_ptr(), n, Gt.data_ptr(), n, 0.0, S.data_ptr(), n, sptr)
+    del G, Gt
+    S.diagonal().add_(1.0)
+    W = torch.empty_like(S)
+    ct = []
+    for it in range(2):
+        W.copy_(S)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        h = C.c_void_p()
+        st = lib.rlg_chol_decompose_f64_dev(n, W.data_ptr(), n, sptr, C.byref(h))
+        e1.record(stream)
+        e1.synchronize()
+        assert st == 0, _lib.last_error()
+        ct.append(e0.elapsed_time(e1))
+        lib.rlg_chol_free(h)
+    tf = n ** 3 / 3 / (min(ct) / 1e3) / 1e12
+    out["cholesky_16384"] = {"ms": round(min(ct), 3), "tflops": round(tf, 2),
+                             "frac_fp64_peak": round(tf / 78.6, 4)}
+    log(f"Cholesky 16384: {min(ct):.1f} ms, {tf:.1f} TF/s")
+    return out
+
+
+def cpu_baseline(cpu_n, n):
+    """rulinalg's PartialPivLu::decompose + solve as restated by the oracle
+    (bit-exact with the Rust code), single core, on a bounded sample."""
+    from oracle import oracle as orc
+    from rulinalg_amd import workloads as wl
+    a = wl.gaussian(cpu_n, cpu_n, 12)
+    b = wl.gaussian(cpu_n, 1, 13)[:, 0]
+    t0 = time.perf_counter()
+    lu, perm, _ = orc.lu_decompose(a)
+    orc.lu_solve(lu, perm, b)
+    dt = time.perf_counter() - t0
+    scale = (n / cpu_n) ** 3
+    log(f"CPU oracle LU n={cpu_n}: {dt:.2f}s -> {dt * scale:.0f}s at n={n}")
+    return {"value": 1.0 / (dt * scale), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"PartialPivLu::decompose + solve, n={cpu_n} N(0,1), {dt:.2f}s, "
+                      f"scaled by (n/{cpu_n})^3 to n={n}",
+            "host": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,771 @@
+// Batched factor-and-solve of many independent systems of uneven order in
+// one persistent launch.
+//
+// Grid: as many workgroups as can be resident (2 per CU); each workgroup
+// takes systems from a device work counter (longest first, the order the
+// host computes) until the list is exhausted, so every workgroup reaches the
+// exit test and the grid drains.  One workgroup factors and solves one
+// system entirely:
+//
+//   LU (PartialPivLu::decompose + solve, lu.rs:163-244): right-looking,
+//   blocked by NB columns.  The (n - k) x NB panel lives in LDS and is
+//   factored with the reference's pivot rule (first maximum of |a|, strict
+//   `>`), division multipliers and separately rounded updates; its swaps are
+//   applied to the rest of the matrix and to the right-hand sides; the block
+//   row is solved in registers; the trailing matrix is updated with
+//   v_mfma_f64_16x16x4f64 tiles whose A-fragments come from the LDS panel.
+//   The right-hand sides ride along as extra columns (forward substitution
+//   fused into the elimination), then a blocked back substitution finishes.
+//
+//   Cholesky (cholesky.rs:116-202): the same skeleton with the reference's
+//   left-looking recurrence inside each diagonal block, lower triangle only,
+//   the SYRK update on lower tiles, and forward / transposed-back
+//   substitutions.
+//
+// NB (8, 16 or 32) is chosen per system from an autotuned table of n-buckets
+// (rlg_batch_autotune); (n + 16) * (NB + 1) doubles must fit the panel
+// buffer.  The mixed form factors an FP32 copy in a per-workgroup workspace
+// and refines in FP64 against the untouched FP64 matrix.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace rlg {
+
+constexpr int BTH = 256;
+constexpr int BMAXN = RLG_BATCH_MAX_N;
+constexpr int PANEL = 8704;       // doubles: 512 x 17
+constexpr int BMAXR = RLG_BATCH_MAX_NRHS;
+
+typedef double bd4 __attribute__((ext_vector_type(4)));
+typedef float bf4 __attribute__((ext_vector_type(4)));
+
+template <class T> struct BMfma;
+template <> struct BMfma<double> {
+    using acc = bd4;
+    static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+    static __device__ __forceinline__ acc op(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct BMfma<float> {
+    using acc = bf4;
+    static __device__ __forceinline__ int row(int lk, int r) { return lk * 4 + r; }
+    static __device__ __forceinline__ acc op(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <class T> __device__ __forceinline__ T beps();
+template <> __device__ __forceinline__ double beps<double>() { return 2.220446049250313e-16; }
+template <> __device__ __forceinline__ float beps<float>() { return 1.1920929e-7f; }
+
+struct Smem {
+    double panel[PANEL];
+    double red_v[BTH / 64];
+    int red_i[BTH / 64];
+    int piv[32];
+    double div[32];
+    int info;
+    int item;
+};
+
+template <class T>
+__device__ __forceinline__ bool bbetter(T v2, int i2, T v1, int i1) {
+    if (i2 < 0) return false;
+    if (i1 < 0) return true;
+    if (v2 > v1) return true;
+    if (v1 > v2) return false;
+    return i2 < i1;
+}
+
+// ---------------------------------------------------------------- LU
+
+// Factor the n x n matrix A (ld = n, type T) in place with RHS vectors B
+// (nrhs x n, may be null) carried along as extra columns, so that on return
+// B holds L^-1 P B.  Pivots go to piv_out[0..n).  Returns the first failing
+// step or n.
+template <class T, int NB>
+__device__ int lu_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
+                        int64_t *__restrict__ piv_out, Smem &sm) {
+    T *P = reinterpret_cast<T *>(sm.panel);
+    constexpr int LD = NB + 1;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) sm.info = n;
+    __syncthreads();
+    for (int k = 0; k < n; k += NB) {
+        const int kb = min(NB, n - k), m = n - k;
+        for (int e = tid; e < m * kb; e += BTH) {
+            const int r = e / kb, c = e - r * kb;
+            P[r * LD + c] = A[(long)(k + r) * n + k + c];
+        }
+        __syncthreads();
+        for (int j = 0; j < kb; ++j) {
+            T bv = T(0);
+            int bi = -1;
+            for (int r = j + tid; r < m; r += BTH) {
+                const T v = fabs(P[r * LD + j]);
+                if (bbetter(v, r, bv, bi)) { bv = v; bi = r; }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const T ov = __shfl_xor(bv, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (bbetter(ov, oi, bv, bi)) { bv = ov; bi = oi; }
+            }
+            if (lane == 0) { sm.red_v[wave] = (double)bv; sm.red_i[wave] = bi; }
+            __syncthreads();
+            if (tid == 0) {
+                T v = (T)sm.red_v[0];
+                int i = sm.red_i[0];
+                for (int q = 1; q < BTH / 64; ++q)
+                    if (bbetter((T)sm.red_v[q], sm.red_i[q], v, i)) { v = (T)sm.red_v[q]; i = sm.red_i[q]; }
+                sm.piv[j] = i;
+                piv_out[k + j] = k + i;
+                if (fabs(P[i * LD + j]) < beps<T>() && sm.info == n) sm.info = k + j;
+            }
+            __syncthreads();
+            const int p = sm.piv[j];
+            if (p != j && tid < kb) {
+                const T t = P[j * LD + tid];
+                P[j * LD + tid] = P[p * LD + tid];
+                P[p * LD + tid] = t;
+            }
+            __syncthreads();
+            {
+#pragma clang fp contract(off)
+                const T piv = P[j * LD + j];
+                for (int r = j + 1 + tid; r < m; r += BTH) {
+                    const T mult = P[r * LD + j] / piv;
+                    P[r * LD + j] = mult;
+                    for (int c = j + 1; c < kb; ++c) P[r * LD + c] = P[r * LD + c] - mult * P[j * LD + c];
+                }
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < m * kb; e += BTH) {
+            const int r = e / kb, c = e - r * kb;
+            A[(long)(k + r) * n + k + c] = P[r * LD + c];
+        }
+        // row swaps on the columns outside the panel and on the RHS
+        const int right = n - k - kb;
+        const int ncols = k + right + nrhs;
+        for (int q = tid; q < ncols; q += BTH) {
+            for (int s = 0; s < kb; ++s) {
+                const int p = sm.piv[s];
+                if (p == s) continue;
+                T *x, *y;
+                if (q < k) {
+                    x = A + (long)(k + s) * n + q;
+                    y = A + (long)(k + p) * n + q;
+                } else if (q < k + right) {
+                    const int c = q - k + k + kb;
+                    x = A + (long)(k + s) * n + c;
+                    y = A + (long)(k + p) * n + c;
+                } else {
+                    const int r = q - k - right;
+                    x = B + (long)r * n + k + s;
+                    y = B + (long)r * n + k + p;
+                }
+                const T t = *x;
+                *x = *y;
+                *y = t;
+            }
+        }
+        __syncthreads();
+        // block row: U12 = L11^-1 A12 and Y1 = L11^-1 B1, one column per thread,
+        // column sweep as gaussian_elimination applies it (lu.rs:603-616)
+        for (int q = tid; q < right + nrhs; q += BTH) {
+            T u[NB];
+            T *base;
+            long stride;
+            if (q < right) { base = A + (long)k * n + k + kb + q; stride = n; }
+            else { base = B + (long)(q - right) * n + k; stride = 1; }
+#pragma unroll
+            for (int i = 0; i < NB; ++i) u[i] = i < kb ? base[i * stride] : T(0);
+            {
+#pragma clang fp contract(off)
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+#pragma unroll
+                    for (int i = c + 1; i < NB; ++i)
+                        if (i < kb) u[i] = u[i] - P[i * LD + c] * u[c];
+            }
+#pragma unroll
+            for (int i = 1; i < NB; ++i)
+                if (i < kb) base[i * stride] = u[i];
+        }
+        __syncthreads();
+        // trailing update A22 -= L21 * U12 on MFMA tiles
+        const int mt = m - kb;  // rows / cols of A22
+        if (mt > 0) {
+            using Acc = typename BMfma<T>::acc;
+            const int tiles = (mt + 15) / 16;
+            const int lr = lane & 15, lk = lane >> 4;
+            for (int t = wave; t < tiles * tiles; t += BTH / 64) {
+                const int ti = t / tiles, tj = t - ti * tiles;
+                const int gj = tj * 16 + lr;  // column within A22
+                Acc acc;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gi = ti * 16 + BMfma<T>::row(lk, r);
+                    acc[r] = (gi < mt && gj < mt) ? A[(long)(k + kb + gi) * n + k + kb + gj] : T(0);
+                }
+#pragma unroll
+                for (int ks = 0; ks < NB; ks += 4) {
+                    const int kk = ks + lk;
+                    const int ai = ti * 16 + lr;
+                    const T af = (ai < mt && kk < kb) ? -P[(kb + ai) * LD + kk] : T(0);
+                    const T bf = (gj < mt && kk < kb) ? A[(long)(k + kk) * n + k + kb + gj] : T(0);
+                    acc = BMfma<T>::op(af, bf, acc);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gi = ti * 16 + BMfma<T>::row(lk, r);
+                    if (gi < mt && gj < mt) A[(long)(k + kb + gi) * n + k + kb + gj] = acc[r];
+                }
+            }
+            // RHS rows below the block: B[r][k+kb+i] -= sum_c L21[i][c] * Y1[r][c]
+            for (int q = tid; q < mt * nrhs; q += BTH) {
+                const int r = q / mt, i = q - r * mt;
+                T *br = B + (long)r * n;
+                T s = br[k + kb + i];
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+                    if (c < kb) s = s - P[(kb + i) * LD + c] * br[k + c];
+                br[k + kb + i] = s;
+            }
+        }
+        __syncthreads();
+    }
+    return sm.info;
+}
+
+enum { TS_LOWER_UNIT = 0, TS_LOWER = 1, TS_UPPER = 2, TS_LOWER_T = 3 };
+
+// In-place triangular solve on nrhs vectors X (vector r at X + r*n) with the
+// triangle of A named by MODE, blocked by 32 rows.  The diagonal block and the
+// block of X are staged in LDS and swept column by column; the remaining
+// rows are updated from contiguous row segments of A.  The sweeps keep the
+// reference's orders inside a block: back_substitution / forward_substitution
+// divide by the stored diagonal, transpose_back_substitution divides and then
+// subtracts x[i] * l[i][j] (cholesky.rs:329-361).
+template <class T, int MODE>
+__device__ void tri_solve(const T *__restrict__ A, int n, T *__restrict__ X, int nrhs, Smem &sm) {
+    constexpr int TB = 32, DL = TB + 1;
+    T *D = reinterpret_cast<T *>(sm.panel);
+    T *XB = D + TB * DL;
+    const int tid = threadIdx.x;
+    const bool fwd = MODE == TS_LOWER_UNIT || MODE == TS_LOWER;
+    const int nblk = (n + TB - 1) / TB;
+    for (int bi = 0; bi < nblk; ++bi) {
+        const int blk = fwd ? bi : nblk - 1 - bi;
+        const int b0 = blk * TB, bs = min(TB, n - b0);
+        for (int e = tid; e < bs * bs; e += BTH) {
+            const int r = e / bs, c = e - r * bs;
+            D[r * DL + c] = A[(long)(b0 + r) * n + b0 + c];
+        }
+        for (int e = tid; e < nrhs * bs; e += BTH) {
+            const int r = e / bs, i = e - r * bs;
+            XB[r * TB + i] = X[(long)r * n + b0 + i];
+        }
+        __syncthreads();
+        {
+#pragma clang fp contract(off)
+            if (fwd) {
+                for (int c = 0; c < bs; ++c) {
+                    if (MODE == TS_LOWER) {
+                        if (tid < nrhs) XB[tid * TB + c] = XB[tid * TB + c] / D[c * DL + c];
+                        __syncthreads();
+                    }
+                    for (int q = tid; q < nrhs * bs; q += BTH) {
+                        const int r = q / bs, i = q - r * bs;
+                        if (i > c) XB[r * TB + i] = XB[r * TB + i] - D[i * DL + c] * XB[r * TB + c];
+                    }
+                    __syncthreads();
+                }
+            } else {
+                for (int c = bs - 1; c >= 0; --c) {
+                    if (tid < nrhs) XB[tid * TB + c] = XB[tid * TB + c] / D[c * DL + c];
+                    __syncthreads();
+                    for (int q = tid; q < nrhs * bs; q += BTH) {
+                        const int r = q / bs, i = q - r * bs;
+                        if (i < c) {
+                            const T l = MODE == TS_UPPER ? D[i * DL + c] : D[c * DL + i];
+                            XB[r * TB + i] = XB[r * TB + i] - (MODE == TS_UPPER ? l * XB[r * TB + c]
+                                                                                : XB[r * TB + c] * l);
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+        for (int e = tid; e < nrhs * bs; e += BTH) {
+            const int r = e / bs, i = e - r * bs;
+            X[(long)r * n + b0 + i] = XB[r * TB + i];
+        }
+        // the rest of the vector
+        const int lo = fwd ? b0 + bs : 0, hi = fwd ? n : b0;
+        const int cnt = hi - lo;
+        for (int q = tid; q < cnt * nrhs; q += BTH) {
+            const int r = q / cnt, i = lo + (q - r * cnt);
+            T s = X[(long)r * n + i];
+            if (MODE == TS_LOWER_T) {
+                for (int c = 0; c < bs; ++c) s = s - A[(long)(b0 + c) * n + i] * XB[r * TB + c];
+            } else {
+                const T *ar = A + (long)i * n + b0;
+                for (int c = 0; c < bs; ++c) s = s - ar[c] * XB[r * TB + c];
+            }
+            X[(long)r * n + i] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- Cholesky
+
+// Left-looking Cholesky of A in place (lower triangle), RHS forward-solved
+// along the way (B := L^-1 B).  Returns the first failing column or n.
+template <class T, int NB>
+__device__ int chol_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs, Smem &sm) {
+    T *P = reinterpret_cast<T *>(sm.panel);
+    constexpr int LD = NB + 1;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) sm.info = n;
+    __syncthreads();
+    for (int k = 0; k < n; k += NB) {
+        const int kb = min(NB, n - k), m = n - k;
+        for (int e = tid; e < m * kb; e += BTH) {
+            const int r = e / kb, c = e - r * kb;
+            if (r >= kb || c <= r) P[r * LD + c] = A[(long)(k + r) * n + k + c];
+        }
+        __syncthreads();
+        {
+#pragma clang fp contract(off)
+            // diagonal block, column by column (cholesky.rs:135-163 within the block)
+            for (int j = 0; j < kb; ++j) {
+                for (int r = j + tid; r < m; r += BTH) {
+                    if (j > 0) {
+                        // utils::dot order over the block's columns 0..j
+                        T p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0, p6 = 0, p7 = 0;
+                        int off = 0;
+                        for (; j - off >= 8; off += 8) {
+                            p0 = p0 + P[r * LD + off + 0] * P[j * LD + off + 0];
+                            p1 = p1 + P[r * LD + off + 1] * P[j * LD + off + 1];
+                            p2 = p2 + P[r * LD + off + 2] * P[j * LD + off + 2];
+                            p3 = p3 + P[r * LD + off + 3] * P[j * LD + off + 3];
+                            p4 = p4 + P[r * LD + off + 4] * P[j * LD + off + 4];
+                            p5 = p5 + P[r * LD + off + 5] * P[j * LD + off + 5];
+                            p6 = p6 + P[r * LD + off + 6] * P[j * LD + off + 6];
+                            p7 = p7 + P[r * LD + off + 7] * P[j * LD + off + 7];
+                        }
+                        T s = T(0);
+                        s = s + p0 + p4;
+                        s = s + p1 + p5;
+                        s = s + p2 + p6;
+                        s = s + p3 + p7;
+                        for (; off < j; ++off) s = s + P[r * LD + off] * P[j * LD + off];
+                        P[r * LD + j] = P[r * LD + j] - s;
+                    }
+                }
+                __syncthreads();
+                const T d = P[j * LD + j];
+                if (tid == 0) {
+                    if ((fabs(d) < beps<T>() || d < T(0)) && sm.info == n) sm.info = k + j;
+                    sm.div[j] = (double)sqrt(d);
+                }
+                __syncthreads();
+                const T dv = (T)sm.div[j];
+                for (int r = j + tid; r < m; r += BTH) P[r * LD + j] = P[r * LD + j] / dv;
+                __syncthreads();
+            }
+        }
+        for (int e = tid; e < m * kb; e += BTH) {
+            const int r = e / kb, c = e - r * kb;
+            if (r >= kb || c <= r) A[(long)(k + r) * n + k + c] = P[r * LD + c];
+        }
+        // RHS: y1 = L11^-1 b1 (forward_substitution order per vector), then
+        // b2 -= L21 y1
+        for (int r = tid; r < nrhs; r += BTH) {
+#pragma clang fp contract(off)
+            T *x = B + (long)r * n + k;
+            for (int i = 0; i < kb; ++i) {
+                T s = T(0);
+                for (int c = 0; c < i; ++c) s = s + P[i * LD + c] * x[c];
+                x[i] = (x[i] - s) / P[i * LD + i];
+            }
+        }
+        __syncthreads();
+        const int mt = m - kb;
+        if (mt > 0) {
+            for (int q = tid; q < mt * nrhs; q += BTH) {
+                const int r = q / mt, i = q - r * mt;
+                T *br = B + (long)r * n;
+                T s = br[k + kb + i];
+                for (int c = 0; c < kb; ++c) s = s - P[(kb + i) * LD + c] * br[k + c];
+                br[k + kb + i] = s;
+            }
+            // SYRK on lower tiles: A22 -= L21 L21^T
+            using Acc = typename BMfma<T>::acc;
+            const int tiles = (mt + 15) / 16;
+            const int lr = lane & 15, lk = lane >> 4;
+            const int ntri = tiles * (tiles + 1) / 2;
+            for (int t = wave; t < ntri; t += BTH / 64) {
+                // t -> (ti, tj) with tj <= ti
+                int ti = (int)((sqrtf(8.f * t + 1.f) - 1.f) * 0.5f);
+                while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+                while (ti * (ti + 1) / 2 > t) --ti;
+                const int tj = t - ti * (ti + 1) / 2;
+                const int gj = tj * 16 + lr;
+                Acc acc;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gi = ti * 16 + BMfma<T>::row(lk, r);
+                    acc[r] = (gi < mt && gj < mt) ? A[(long)(k + kb + gi) * n + k + kb + gj] : T(0);
+                }
+#pragma unroll
+                for (int ks = 0; ks < NB; ks += 4) {
+                    const int kk = ks + lk;
+                    const int ai = ti * 16 + lr;
+                    const T af = (ai < mt && kk < kb) ? -P[(kb + ai) * LD + kk] : T(0);
+                    const T bf = (gj < mt && kk < kb) ? P[(kb + gj) * LD + kk] : T(0);
+                    acc = BMfma<T>::op(af, bf, acc);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gi = ti * 16 + BMfma<T>::row(lk, r);
+                    if (gi < mt && gj < mt && gj <= gi) A[(long)(k + kb + gi) * n + k + kb + gj] = acc[r];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    return sm.info;
+}
+
+// ---------------------------------------------------------------- dispatch
+
+template <class T>
+__device__ int solve_system(int kind, int nb, T *A, int n, T *B, int nrhs, int64_t *ipiv, Smem &sm) {
+    int info;
+    if (kind == RLG_KIND_LU) {
+        if (nb == 8) info = lu_fused<T, 8>(A, n, B, nrhs, ipiv, sm);
+        else if (nb == 32) info = lu_fused<T, 32>(A, n, B, nrhs, ipiv, sm);
+        else info = lu_fused<T, 16>(A, n, B, nrhs, ipiv, sm);
+        if (info == n) tri_solve<T, TS_UPPER>(A, n, B, nrhs, sm);
+    } else {
+        if (nb == 8) info = chol_fused<T, 8>(A, n, B, nrhs, sm);
+        else if (nb == 32) info = chol_fused<T, 32>(A, n, B, nrhs, sm);
+        else info = chol_fused<T, 16>(A, n, B, nrhs, sm);
+        if (info == n) tri_solve<T, TS_LOWER_T>(A, n, B, nrhs, sm);
+    }
+    return info;
+}
+
+__global__ __launch_bounds__(BTH) void k_batch_f64(rlg_batch_item *__restrict__ items,
+                                                   const int *__restrict__ order,
+                                                   const unsigned char *__restrict__ nbs, int count,
+                                                   unsigned *__restrict__ counter) {
+    __shared__ Smem sm;
+    for (;;) {
+        if (threadIdx.x == 0) sm.item = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int idx = sm.item;
+        __syncthreads();
+        if (idx >= count) break;
+        const int id = order[idx];
+        rlg_batch_item it = items[id];
+        const int info = solve_system<double>(it.kind, nbs[id], it.a, it.n, it.b, it.nrhs, it.ipiv, sm);
+        if (threadIdx.x == 0)
+            items[id].status = info == it.n ? RLG_OK
+                               : (it.kind == RLG_KIND_LU ? RLG_E_DIV_BY_ZERO : RLG_E_DECOMP_FAILURE);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- mixed
+
+// Per-workgroup workspace for the mixed form: FP32 factor (n*n), FP32
+// correction / pivots, FP64 residual and iterate, FP64 fallback copy.
+struct MixedSlot {
+    float *a32;      // BMAXN * BMAXN
+    float *d32;      // BMAXR * BMAXN
+    double *res;     // BMAXR * BMAXN
+    double *x;       // BMAXR * BMAXN
+    double *a64;     // BMAXN * BMAXN (fallback)
+    int64_t *piv;    // BMAXN
+};
+constexpr size_t SLOT_BYTES = (size_t)BMAXN * BMAXN * 4 + (size_t)BMAXR * BMAXN * 4 +
+                              2 * (size_t)BMAXR * BMAXN * 8 + (size_t)BMAXN * BMAXN * 8 + BMAXN * 8;
+
+__device__ MixedSlot slot_of(char *ws, int s) {
+    char *p = ws + (size_t)s * SLOT_BYTES;
+    MixedSlot m;
+    m.a32 = reinterpret_cast<float *>(p); p += (size_t)BMAXN * BMAXN * 4;
+    m.d32 = reinterpret_cast<float *>(p); p += (size_t)BMAXR * BMAXN * 4;
+    m.res = reinterpret_cast<double *>(p); p += (size_t)BMAXR * BMAXN * 8;
+    m.x = reinterpret_cast<double *>(p); p += (size_t)BMAXR * BMAXN * 8;
+    m.a64 = reinterpret_cast<double *>(p); p += (size_t)BMAXN * BMAXN * 8;
+    m.piv = reinterpret_cast<int64_t *>(p);
+    return m;
+}
+
+// max over i of |v[i]| (all threads return the same value)
+__device__ double block_maxabs_sum(double v, Smem &sm, bool is_max) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = is_max ? fmax(v, o) : v + o;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sm.red_v[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = sm.red_v[0];
+    for (int q = 1; q < BTH / 64; ++q) r = is_max ? fmax(r, sm.red_v[q]) : r + sm.red_v[q];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(BTH) void k_batch_mixed(rlg_batch_item *__restrict__ items,
+                                                     const int *__restrict__ order,
+                                                     const unsigned char *__restrict__ nbs, int count,
+                                                     unsigned *__restrict__ counter, char *ws,
+                                                     int max_iter, double tol,
+                                                     unsigned *__restrict__ fallbacks) {
+    __shared__ Smem sm;
+    const MixedSlot ms = slot_of(ws, blockIdx.x);
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) sm.item = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int idx = sm.item;
+        __syncthreads();
+        if (idx >= count) break;
+        const int id = order[idx];
+        rlg_batch_item it = items[id];
+        const int n = it.n, nrhs = it.nrhs;
+        const double *A = it.a;
+        // FP32 copy, ||A||inf
+        double rs = 0.0, anorm = 0.0;
+        for (int e = tid; e < n * n; e += BTH) ms.a32[e] = (float)A[e];
+        for (int i = tid; i < n; i += BTH) {
+            double s = 0.0;
+            for (int j = 0; j < n; ++j) s += fabs(A[(long)i * n + j]);
+            rs = fmax(rs, s);
+        }
+        anorm = block_maxabs_sum(rs, sm, true);
+        __syncthreads();
+        int info;
+        if (it.kind == RLG_KIND_LU) {
+            const int nb = nbs[id];
+            if (nb == 8) info = lu_fused<float, 8>(ms.a32, n, nullptr, 0, ms.piv, sm);
+            else if (nb == 32) info = lu_fused<float, 32>(ms.a32, n, nullptr, 0, ms.piv, sm);
+            else info = lu_fused<float, 16>(ms.a32, n, nullptr, 0, ms.piv, sm);
+        } else {
+            const int nb = nbs[id];
+            if (nb == 8) info = chol_fused<float, 8>(ms.a32, n, nullptr, 0, sm);
+            else if (nb == 32) info = chol_fused<float, 32>(ms.a32, n, nullptr, 0, sm);
+            else info = chol_fused<float, 16>(ms.a32, n, nullptr, 0, sm);
+        }
+        // row permutation of the FP32 factor: raw[k] = original row at k
+        int *raw = reinterpret_cast<int *>(sm.panel + 4096);
+        if (tid == 0 && it.kind == RLG_KIND_LU && info == n) {
+            for (int i = 0; i < n; ++i) raw[i] = i;
+            for (int k = 0; k < n; ++k) {
+                const int p = (int)ms.piv[k];
+                if (p != k) { const int t = raw[k]; raw[k] = raw[p]; raw[p] = t; }
+            }
+        }
+        __syncthreads();
+        bool converged = false;
+        if (info == n) {
+            for (int r = 0; r < nrhs; ++r) {
+                double *x = ms.x + (long)r * n;
+                double *res = ms.res + (long)r * n;
+                float *d = ms.d32 + (long)r * n;
+                const double *b = it.b + (long)r * n;
+                double bn = 0.0;
+                for (int i = tid; i < n; i += BTH) { x[i] = 0.0; res[i] = b[i]; bn = fmax(bn, fabs(b[i])); }
+                const double bnorm = block_maxabs_sum(bn, sm, true);
+                double prev = INFINITY, eta = INFINITY;
+                for (int itn = 0; itn < max_iter; ++itn) {
+                    if (it.kind == RLG_KIND_LU) {
+                        for (int i = tid; i < n; i += BTH) d[i] = (float)res[raw[i]];
+                        __syncthreads();
+                        tri_solve<float, TS_LOWER_UNIT>(ms.a32, n, d, 1, sm);
+                        tri_solve<float, TS_UPPER>(ms.a32, n, d, 1, sm);
+                    } else {
+                        for (int i = tid; i < n; i += BTH) d[i] = (float)res[i];
+                        __syncthreads();
+                        tri_solve<float, TS_LOWER>(ms.a32, n, d, 1, sm);
+                        tri_solve<float, TS_LOWER_T>(ms.a32, n, d, 1, sm);
+                    }
+                    double xn = 0.0;
+                    for (int i = tid; i < n; i += BTH) { x[i] += (double)d[i]; xn = fmax(xn, fabs(x[i])); }
+                    const double xnorm = block_maxabs_sum(xn, sm, true);
+                    double rn = 0.0;
+                    for (int i = tid; i < n; i += BTH) {
+                        const double *ar = A + (long)i * n;
+                        double s = 0.0;
+                        for (int j = 0; j < n; ++j) s += ar[j] * x[j];
+                        res[i] = b[i] - s;
+                        rn = fmax(rn, fabs(res[i]));
+                    }
+                    const double rnorm = block_maxabs_sum(rn, sm, true);
+                    const double den = anorm * xnorm + bnorm;
+                    eta = den > 0 ? rnorm / den : 0.0;
+                    if (eta <= tol) break;
+                    if (itn >= 1 && !(eta <= 0.5 * prev)) break;
+                    prev = eta;
+                }
+                converged = eta <= tol;
+                if (!converged) break;
+            }
+        }
+        if (converged) {
+            for (int e = tid; e < nrhs * n; e += BTH) it.b[e] = ms.x[e];
+            if (tid == 0) items[id].status = RLG_OK;
+        } else {
+            // fallback: plain FP64 on a copy of A
+            for (int e = tid; e < n * n; e += BTH) ms.a64[e] = A[e];
+            __syncthreads();
+            const int inf64 = solve_system<double>(it.kind, nbs[id], ms.a64, n, it.b, nrhs, ms.piv, sm);
+            if (tid == 0) {
+                items[id].status = inf64 == n ? RLG_OK
+                                   : (it.kind == RLG_KIND_LU ? RLG_E_DIV_BY_ZERO : RLG_E_DECOMP_FAILURE);
+                atomicAdd(fallbacks, 1u);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- host side
+
+namespace {
+
+// Autotuned panel width per n-bucket (n <= 32, 64, 128, 256, 512).
+std::mutex g_tune_mu;
+int g_nb_table[5] = {8, 16, 16, 16, 16};
+
+int bucket_of(int n) {
+    if (n <= 32) return 0;
+    if (n <= 64) return 1;
+    if (n <= 128) return 2;
+    if (n <= 256) return 3;
+    return 4;
+}
+
+int nb_for(int n) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    int nb = g_nb_table[bucket_of(n)];
+    if (nb == 32 && (n + 16) * 33 > PANEL) nb = 16;
+    return nb;
+}
+
+int resident_grid(Device *d) { return 2 * d->num_cus; }
+
+struct Prepared {
+    PoolBuf order, nbs, counter;
+    int count = 0;
+};
+
+int prepare(hipStream_t s, size_t count, rlg_batch_item *items, Prepared &p) {
+    std::vector<rlg_batch_item> h(count);
+    RLG_HIP(hipMemcpyAsync(h.data(), items, count * sizeof(rlg_batch_item), hipMemcpyDeviceToHost, s));
+    RLG_HIP(hipStreamSynchronize(s));
+    std::vector<int> order(count);
+    std::vector<unsigned char> nbs(count);
+    for (size_t i = 0; i < count; ++i) {
+        const auto &it = h[i];
+        if (it.n < 1 || it.n > BMAXN || it.nrhs < 0 || it.nrhs > BMAXR || !it.a ||
+            (it.nrhs > 0 && !it.b) || (it.kind != RLG_KIND_LU && it.kind != RLG_KIND_CHOL) ||
+            (it.kind == RLG_KIND_LU && !it.ipiv)) {
+            set_error("batch item %zu is invalid (kind %d, n %d, nrhs %d)", i, it.kind, it.n, it.nrhs);
+            return RLG_E_INVALID_ARG;
+        }
+        order[i] = (int)i;
+        nbs[i] = (unsigned char)nb_for(it.n);
+    }
+    // longest first: cost ~ n^3 (Cholesky about half of LU)
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const double cx = (double)h[x].n * h[x].n * h[x].n * (h[x].kind == RLG_KIND_LU ? 2 : 1);
+        const double cy = (double)h[y].n * h[y].n * h[y].n * (h[y].kind == RLG_KIND_LU ? 2 : 1);
+        return cx > cy;
+    });
+    RLG_TRY(p.order.alloc(count * sizeof(int), s));
+    RLG_TRY(p.nbs.alloc(count, s));
+    RLG_TRY(p.counter.alloc(sizeof(unsigned) * 2, s));
+    RLG_HIP(hipMemcpyAsync(p.order.p, order.data(), count * sizeof(int), hipMemcpyHostToDevice, s));
+    RLG_HIP(hipMemcpyAsync(p.nbs.p, nbs.data(), count, hipMemcpyHostToDevice, s));
+    RLG_TRY(fill_u32(p.counter.as<uint32_t>(), 2, 0u, s));
+    RLG_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
+    p.count = (int)count;
+    return RLG_OK;
+}
+
+}  // namespace
+
+int batch_solve_f64(hipStream_t s, Device *d, size_t count, rlg_batch_item *items) {
+    if (count == 0) return RLG_OK;
+    Prepared p;
+    RLG_TRY(prepare(s, count, items, p));
+    const int grid = std::min<int>(resident_grid(d), (int)count);
+    RLG_PROF("batch_f64", s);
+    hipLaunchKernelGGL(k_batch_f64, dim3(grid), dim3(BTH), 0, s, items, p.order.as<int>(),
+                       p.nbs.as<unsigned char>(), p.count, p.counter.as<unsigned>());
+    RLG_LAUNCHED();
+    return RLG_OK;
+}
+
+int batch_solve_mixed(hipStream_t s, Device *d, size_t count, rlg_batch_item *items, int max_iter,
+                      double tol, size_t *fallbacks) {
+    if (count == 0) { if (fallbacks) *fallbacks = 0; return RLG_OK; }
+    Prepared p;
+    RLG_TRY(prepare(s, count, items, p));
+    const int grid = std::min<int>(resident_grid(d), (int)count);
+    PoolBuf ws;
+    RLG_TRY(ws.alloc(SLOT_BYTES * grid, s));
+    {
+        RLG_PROF("batch_mixed", s);
+        hipLaunchKernelGGL(k_batch_mixed, dim3(grid), dim3(BTH), 0, s, items, p.order.as<int>(),
+                           p.nbs.as<unsigned char>(), p.count, p.counter.as<unsigned>(), ws.as<char>(),
+                           max_iter, tol, p.counter.as<unsigned>() + 1);
+        RLG_LAUNCHED();
+    }
+    unsigned fb = 0;
+    RLG_HIP(hipMemcpyAsync(&fb, p.counter.as<unsigned>() + 1, 4, hipMemcpyDeviceToHost, s));
+    RLG_HIP(hipStreamSynchronize(s));
+    if (fallbacks) *fallbacks = fb;
+    return RLG_OK;
+}
+
+void batch_set_nb(int bucket, int nb) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (bucket >= 0 && bucket < 5 && (nb == 8 || nb == 16 || nb == 32)) g_nb_table[bucket] = nb;
+}
+
+}  // namespace rlg
+
+using namespace rlg;
+
+extern "C" {
+
+int rlg_batch_solve_f64_dev(size_t count, rlg_batch_item *items, void *stream) {
+    Device *d;
+    RLG_TRY(current_device(&d));
+    return batch_solve_f64(pick_stream(d, stream), d, count, items);
+}
+
+int rlg_batch_solve_mixed_dev(size_t count, rlg_batch_item *items, int max_iter, double tol,
+                              size_t *fallbacks, void *stream) {
+    Device *d;
+    RLG_TRY(current_device(&d));
+    if (max_iter < 1) max_iter = 30;
+    if (!(tol > 0)) tol = 1e-13;
+    return batch_solve_mixed(pick_stream(d, stream), d, count, items, max_iter, tol, fallbacks);
+}
+
+}  // extern "C"

@@ -1,0 +1,189 @@
+// Blocked Cholesky (Cholesky::decompose, src/matrix/decomposition/cholesky.rs:
+// 116-169), lower triangle only, and its solve (cholesky.rs:194-202).
+//
+// Outer blocks of CONB = 256 columns end in one SYRK-shaped MFMA GEMM of
+// depth 256 on the lower tiles of the trailing matrix.  Inside an outer
+// block, 64-column blocks are processed left to right:
+//   1. the diagonal block is factored by one workgroup in LDS with the
+//      reference's left-looking "gaxpy" recurrence: a[k][j] -= dot(row k,
+//      row j) over the block's previous columns in utils::dot's eight-lane
+//      order, the |d| < eps and d < 0 checks, and division by sqrt(d);
+//   2. the rows below are solved against it with the same recurrence
+//      (a triangular solve X L^T = B, row by row);
+//   3. a depth-64 GEMM updates the remaining columns of the outer block.
+// A matrix of order <= 64 is therefore factored bit-identically to the
+// reference.  Only entries on or below the diagonal are read; entries above
+// it may be overwritten with intermediate values (the reference's unpack
+// zeroes them, cholesky.rs:237-246).
+#include "kernels.hpp"
+
+namespace rlg {
+
+constexpr int CNB = 64;
+constexpr int CONB = 256;
+
+template <class T> __device__ __forceinline__ T ceps();
+template <> __device__ __forceinline__ double ceps<double>() { return 2.220446049250313e-16; }
+template <> __device__ __forceinline__ float ceps<float>() { return 1.1920929e-7f; }
+
+// utils::dot (src/utils.rs:20-51) of u[0..len) and v[0..len) where u has
+// unit stride and v has stride vs.
+template <class T>
+__device__ __forceinline__ T dot8(const T *u, const T *v, int vs, int len) {
+#pragma clang fp contract(off)
+    T p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0, p6 = 0, p7 = 0;
+    int off = 0;
+    for (; len - off >= 8; off += 8) {
+        p0 = p0 + u[off + 0] * v[(off + 0) * vs];
+        p1 = p1 + u[off + 1] * v[(off + 1) * vs];
+        p2 = p2 + u[off + 2] * v[(off + 2) * vs];
+        p3 = p3 + u[off + 3] * v[(off + 3) * vs];
+        p4 = p4 + u[off + 4] * v[(off + 4) * vs];
+        p5 = p5 + u[off + 5] * v[(off + 5) * vs];
+        p6 = p6 + u[off + 6] * v[(off + 6) * vs];
+        p7 = p7 + u[off + 7] * v[(off + 7) * vs];
+    }
+    T s = T(0);
+    s = s + p0 + p4;
+    s = s + p1 + p5;
+    s = s + p2 + p6;
+    s = s + p3 + p7;
+    for (; off < len; ++off) s = s + u[off] * v[off * vs];
+    return s;
+}
+
+// Factor the jb x jb diagonal block at (j0, j0).  divisor[j0 + j] receives
+// sqrt(d_j), the value the reference divides the column by.
+template <class T>
+__global__ __launch_bounds__(64) void k_potrf_diag(T *__restrict__ A, long lda, int j0, int jb,
+                                                   T *__restrict__ divisor,
+                                                   int64_t *__restrict__ info, int n) {
+#pragma clang fp contract(off)
+    __shared__ T a[CNB][CNB + 1];
+    const int t = threadIdx.x;
+    for (int r = 0; r < jb; ++r)
+        if (t <= r) a[r][t] = A[(long)(j0 + r) * lda + j0 + t];
+    __syncthreads();
+    bool failed = false;
+    for (int j = 0; j < jb; ++j) {
+        // a[k][j] -= dot(a[k][0..j], a[j][0..j]) for k >= j (cholesky.rs:141-148)
+        if (t >= j && t < jb && j > 0) {
+            const T d = dot8(&a[t][0], &a[j][0], 1, j);
+            a[t][j] = a[t][j] - d;
+        }
+        __syncthreads();
+        const T diag = a[j][j];
+        if (fabs(diag) < ceps<T>() || diag < T(0)) {
+            if (t == 0 && *info == (int64_t)n) *info = j0 + j;
+            failed = true;
+        }
+        const T div = sqrt(diag);
+        if (t >= j && t < jb) a[t][j] = a[t][j] / div;
+        if (t == 0) divisor[j0 + j] = div;
+        __syncthreads();
+        if (failed) break;  // the reference stops at the first failure
+    }
+    for (int r = 0; r < jb; ++r)
+        if (t <= r) A[(long)(j0 + r) * lda + j0 + t] = a[r][t];
+}
+
+// Rows i in [r0, n) of the block column j0..j0+jb: solve x L11^T = b with
+// the same recurrence as the diagonal block: x[c] = (b[c] - dot(x[0..c],
+// L11[c][0..c])) / divisor[c].  One thread per row, 64 rows per workgroup.
+template <class T>
+__global__ __launch_bounds__(64) void k_chol_trsm(T *__restrict__ A, long lda, int j0, int jb, int r0,
+                                                  int n, const T *__restrict__ divisor) {
+#pragma clang fp contract(off)
+    __shared__ T L[CNB][CNB + 1];
+    __shared__ T X[64][CNB + 1];
+    const int t = threadIdx.x;
+    const int row = r0 + blockIdx.x * 64 + t;
+    for (int r = 0; r < jb; ++r)
+        if (t <= r) L[r][t] = A[(long)(j0 + r) * lda + j0 + t];
+    const bool act = row < n;
+    for (int c = 0; c < jb; ++c) X[t][c] = act ? A[(long)row * lda + j0 + c] : T(0);
+    __syncthreads();
+    if (act) {
+        for (int c = 0; c < jb; ++c) {
+            const T d = c ? dot8(&X[t][0], &L[c][0], 1, c) : T(0);
+            X[t][c] = (X[t][c] - d) / divisor[j0 + c];
+        }
+        for (int c = 0; c < jb; ++c) A[(long)row * lda + j0 + c] = X[t][c];
+    }
+}
+
+size_t chol_work_bytes(size_t n, size_t elem) { return ((n + 1) * elem + 255) & ~(size_t)255; }
+
+template <class T>
+static int chol_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info, void *work,
+                     int lowp_prec) {
+    T *divisor = static_cast<T *>(work);
+    RLG_TRY(fill_i64(info, 1, (int64_t)n, s));
+    for (size_t k0 = 0; k0 < n; k0 += CONB) {
+        const size_t kb = n - k0 < (size_t)CONB ? n - k0 : CONB;
+        for (size_t j0 = k0; j0 < k0 + kb; j0 += CNB) {
+            const size_t jb = k0 + kb - j0 < (size_t)CNB ? k0 + kb - j0 : CNB;
+            {
+                RLG_PROF("chol_diag", s);
+                hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(64), 0, s, a, (long)lda, (int)j0,
+                                   (int)jb, divisor, info, (int)n);
+                RLG_LAUNCHED();
+            }
+            const size_t r0 = j0 + jb;
+            if (r0 < n) {
+                RLG_PROF("chol_trsm", s);
+                hipLaunchKernelGGL(k_chol_trsm<T>, dim3(ceil_div(n - r0, 64)), dim3(64), 0, s, a,
+                                   (long)lda, (int)j0, (int)jb, (int)r0, (int)n, divisor);
+                RLG_LAUNCHED();
+            }
+            if (r0 < k0 + kb) {
+                // remaining columns of the outer block, all rows below the diagonal
+                RLG_TRY(gemm_t<T>(s, n - r0, k0 + kb - r0, jb, T(-1), a + r0 * lda + j0, lda,
+                                  a + r0 * lda + j0, lda, T(1), a + r0 * lda + r0, lda, true, false));
+            }
+        }
+        const size_t c0 = k0 + kb;
+        if (c0 < n) {
+            if (lowp_prec == RLG_PREC_F16 || lowp_prec == RLG_PREC_BF16) {
+                RLG_TRY(gemm_lowp(s, lowp_prec, n - c0, n - c0, kb, -1.f,
+                                  reinterpret_cast<const float *>(a + c0 * lda + k0), lda,
+                                  reinterpret_cast<const float *>(a + c0 * lda + k0), lda, 1.f,
+                                  reinterpret_cast<float *>(a + c0 * lda + c0), lda, true, true));
+            } else {
+                RLG_TRY(gemm_t<T>(s, n - c0, n - c0, kb, T(-1), a + c0 * lda + k0, lda,
+                                  a + c0 * lda + k0, lda, T(1), a + c0 * lda + c0, lda, true, true));
+            }
+        }
+    }
+    return RLG_OK;
+}
+
+template <class T>
+int chol_factor(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info, void *work) {
+    return chol_impl<T>(s, n, a, lda, info, work, RLG_PREC_F64);
+}
+template int chol_factor<double>(hipStream_t, size_t, double *, size_t, int64_t *, void *);
+template int chol_factor<float>(hipStream_t, size_t, float *, size_t, int64_t *, void *);
+
+int chol_factor_lowp(hipStream_t s, int prec, size_t n, float *a, size_t lda, int64_t *info,
+                     void *work) {
+    return chol_impl<float>(s, n, a, lda, info, work, prec);
+}
+
+template <class T>
+int chol_solve(hipStream_t s, size_t n, const T *l, size_t lda, size_t nrhs, const T *b, size_t ldb,
+               T *x, size_t ldx) {
+    if (n == 0 || nrhs == 0) return RLG_OK;
+    if (x != b)
+        for (size_t r = 0; r < nrhs; ++r)
+            RLG_HIP(hipMemcpyAsync(x + r * ldx, b + r * ldb, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    RLG_TRY(trsv_lower<T>(s, n, l, lda, false, nrhs, x, ldx));
+    RLG_TRY(trsv_lower_t<T>(s, n, l, lda, nrhs, x, ldx));
+    return RLG_OK;
+}
+template int chol_solve<double>(hipStream_t, size_t, const double *, size_t, size_t, const double *,
+                                size_t, double *, size_t);
+template int chol_solve<float>(hipStream_t, size_t, const float *, size_t, size_t, const float *,
+                               size_t, float *, size_t);
+
+}  // namespace rlg

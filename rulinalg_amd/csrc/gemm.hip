@@ -1,0 +1,400 @@
+// GEMM on the gfx950 matrix cores.
+//
+// One kernel family serves `&a * &b`, the LU Schur update, the Cholesky
+// SYRK update and the mixed-precision updates:
+//   C = alpha * A * op(B) + beta * C, all row-major with row strides.
+//
+// Tiling (FP64/FP32): a 256-thread workgroup owns a 128 x 128 tile of C and
+// walks K in steps of 16, double-buffered through LDS.  The four wave64s
+// are arranged 2 x 2; each owns a 64 x 64 sub-tile held as 4 x 4 MFMA
+// accumulators of 16 x 16 (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
+// A is staged k-major (As[k][i]) so that an MFMA A-fragment (lane l holds
+// A[l%16][l/16]) is four contiguous 128-byte row segments; B is staged
+// As-is (Bs[k][j]).  The +1 padding of transposed stages spreads the
+// transposing LDS writes over all banks.
+//
+// Low precision (FP16/BF16): operands stay FP32 in memory and are rounded on
+// their way into LDS; v_mfma_f32_16x16x32_{f16,bf16} accumulate in FP32.
+// Fragments hold 8 consecutive k per lane, so both stages are k-contiguous
+// ([i][k] and [j][k]).
+#include "kernels.hpp"
+
+namespace rlg {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+
+template <class T> struct Mfma;
+// Accumulator layouts of the 16 x 16 result: for the FP32 instruction lane l,
+// register r holds D[4*(l/16) + r][l%16]; for the FP64 instruction it holds
+// D[(l/16) + 4*r][l%16].
+template <> struct Mfma<double> {
+    using acc = d4;
+    static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+    static __device__ __forceinline__ acc op(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    using acc = f4;
+    static __device__ __forceinline__ int row(int lk, int r) { return lk * 4 + r; }
+    static __device__ __forceinline__ acc op(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+constexpr int GBM = 128, GBN = 128, GBK = 16, GTHREADS = 256;
+
+// Maps the linear workgroup id onto (tile_m, tile_n).  Consecutive ids are
+// dispatched round-robin over the 8 XCDs; regrouping gives each XCD a
+// contiguous range of tiles, and within a range tiles go down columns of
+// GROUP_M tile-rows, so that an XCD's L2 sees reuse of both A and B panels.
+__device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, int &tn) {
+    const int total = tiles_m * tiles_n;
+    int id = blockIdx.x;
+    if (total >= 64 && (total % 8) == 0) {
+        const int per_xcd = total / 8;
+        id = (id % 8) * per_xcd + id / 8;
+    }
+    constexpr int GROUP_M = 8;
+    const int group_size = GROUP_M * tiles_n;
+    const int g = id / group_size;
+    const int first_m = g * GROUP_M;
+    const int gm = min(tiles_m - first_m, GROUP_M);
+    const int r = id - g * group_size;
+    tm = first_m + r % gm;
+    tn = r / gm;
+}
+
+template <class T, bool TRANSB, bool LOWER>
+__global__ __launch_bounds__(GTHREADS) void k_gemm(
+    int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
+    const T *__restrict__ B, long ldb, T beta, T *__restrict__ C, long ldc,
+    int tiles_m, int tiles_n) {
+    using Acc = typename Mfma<T>::acc;
+    __shared__ T As[2][GBK][GBM + 1];
+    __shared__ T Bs[2][GBK][GBN + (TRANSB ? 1 : 0)];
+
+    int tm, tn;
+    tile_coords(tiles_m, tiles_n, tm, tn);
+    if (LOWER && tn > tm) return;  // strictly above the diagonal
+    const int m0 = tm * GBM, n0 = tn * GBN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    Acc acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+
+    T ra[8], rb[8];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = q * GTHREADS + tid;
+            {   // A tile, 128 x 16: e -> (i = e / 16, kk = e % 16)
+                const int i = e >> 4, kk = e & 15;
+                const int gi = m0 + i, gk = k0 + kk;
+                ra[q] = (gi < M && gk < K) ? A[(long)gi * lda + gk] : T(0);
+            }
+            if (!TRANSB) {  // B tile, 16 x 128: e -> (kk = e / 128, j = e % 128)
+                const int kk = e >> 7, j = e & 127;
+                const int gk = k0 + kk, gj = n0 + j;
+                rb[q] = (gk < K && gj < N) ? B[(long)gk * ldb + gj] : T(0);
+            } else {        // B^T given as N x K: e -> (j = e / 16, kk = e % 16)
+                const int j = e >> 4, kk = e & 15;
+                const int gj = n0 + j, gk = k0 + kk;
+                rb[q] = (gj < N && gk < K) ? B[(long)gj * ldb + gk] : T(0);
+            }
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = q * GTHREADS + tid;
+            As[buf][e & 15][e >> 4] = ra[q];
+            if (!TRANSB)
+                Bs[buf][e >> 7][e & 127] = rb[q];
+            else
+                Bs[buf][e & 15][e >> 4] = rb[q];
+        }
+    };
+
+    const int KT = (K + GBK - 1) / GBK;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < KT) load((kt + 1) * GBK);
+#pragma unroll
+        for (int ks = 0; ks < GBK / 4; ++ks) {
+            T af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) af[i] = As[buf][ks * 4 + lk][wm + i * 16 + lr];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bf[j] = Bs[buf][ks * 4 + lk][wn + j * 16 + lr];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(af[i], bf[j], acc[i][j]);
+        }
+        if (kt + 1 < KT) store(buf ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wm + i * 16 + Mfma<T>::row(lk, r);
+                const int gj = n0 + wn + j * 16 + lr;
+                if (gi < M && gj < N) {
+                    T *p = C + (long)gi * ldc + gj;
+                    T v = alpha * acc[i][j][r];
+                    if (beta != T(0)) v += beta * *p;
+                    *p = v;
+                }
+            }
+}
+
+// ---------------------------------------------------------------- low precision
+
+template <class H> struct LowpMfma;
+template <> struct LowpMfma<_Float16> {
+    using vec = h8;
+    static __device__ __forceinline__ f4 op(vec a, vec b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ _Float16 cvt(float x) { return (_Float16)x; }
+};
+template <> struct LowpMfma<__bf16> {
+    using vec = b8;
+    static __device__ __forceinline__ f4 op(vec a, vec b, f4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ __bf16 cvt(float x) { return (__bf16)x; }
+};
+
+constexpr int LBK = 64;          // k per stage (two MFMA k-steps of 32)
+constexpr int LPAD = 8;          // halves of padding per staged row
+
+template <class H, bool TRANSB, bool LOWER>
+__global__ __launch_bounds__(GTHREADS) void k_gemm_lowp(
+    int M, int N, int K, float alpha, const float *__restrict__ A, long lda,
+    const float *__restrict__ B, long ldb, float beta, float *__restrict__ C,
+    long ldc, int tiles_m, int tiles_n) {
+    using V = typename LowpMfma<H>::vec;
+    __shared__ __attribute__((aligned(16))) H As[2][GBM][LBK + LPAD];
+    __shared__ __attribute__((aligned(16))) H Bs[2][GBN][LBK + LPAD];
+
+    int tm, tn;
+    tile_coords(tiles_m, tiles_n, tm, tn);
+    if (LOWER && tn > tm) return;
+    const int m0 = tm * GBM, n0 = tn * GBN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4{0, 0, 0, 0};
+
+    // 128 x 64 elements per operand per stage = 32 per thread.
+    float ra[32], rb[32];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const int e = q * GTHREADS + tid;
+            {   // A: e -> (i = e / 64, kk = e % 64)
+                const int i = e >> 6, kk = e & 63;
+                const int gi = m0 + i, gk = k0 + kk;
+                ra[q] = (gi < M && gk < K) ? A[(long)gi * lda + gk] : 0.f;
+            }
+            if (!TRANSB) {  // B: e -> (kk = e / 128, j = e % 128)
+                const int kk = e >> 7, j = e & 127;
+                const int gk = k0 + kk, gj = n0 + j;
+                rb[q] = (gk < K && gj < N) ? B[(long)gk * ldb + gj] : 0.f;
+            } else {
+                const int j = e >> 6, kk = e & 63;
+                const int gj = n0 + j, gk = k0 + kk;
+                rb[q] = (gj < N && gk < K) ? B[(long)gj * ldb + gk] : 0.f;
+            }
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const int e = q * GTHREADS + tid;
+            As[buf][e >> 6][e & 63] = LowpMfma<H>::cvt(ra[q]);
+            if (!TRANSB)
+                Bs[buf][e & 127][e >> 7] = LowpMfma<H>::cvt(rb[q]);
+            else
+                Bs[buf][e >> 6][e & 63] = LowpMfma<H>::cvt(rb[q]);
+        }
+    };
+
+    const int KT = (K + LBK - 1) / LBK;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < KT) load((kt + 1) * LBK);
+#pragma unroll
+        for (int ks = 0; ks < LBK / 32; ++ks) {
+            V af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                af[i] = *reinterpret_cast<const V *>(&As[buf][wm + i * 16 + lr][ks * 32 + lk * 8]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                bf[j] = *reinterpret_cast<const V *>(&Bs[buf][wn + j * 16 + lr][ks * 32 + lk * 8]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = LowpMfma<H>::op(af[i], bf[j], acc[i][j]);
+        }
+        if (kt + 1 < KT) store(buf ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wm + i * 16 + lk * 4 + r;
+                const int gj = n0 + wn + j * 16 + lr;
+                if (gi < M && gj < N) {
+                    float *p = C + (long)gi * ldc + gj;
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.f) v += beta * *p;
+                    *p = v;
+                }
+            }
+}
+
+// ---------------------------------------------------------------- launchers
+
+static int check_dims(size_t m, size_t n, size_t k) {
+    if (m > (size_t)INT32_MAX / 2 || n > (size_t)INT32_MAX / 2 || k > (size_t)INT32_MAX / 2) {
+        set_error("gemm dimension too large (%zu, %zu, %zu)", m, n, k);
+        return RLG_E_INVALID_ARG;
+    }
+    return RLG_OK;
+}
+
+template <class T>
+static int gemm_impl(hipStream_t s, const char *cls, size_t m, size_t n, size_t k, T alpha,
+                     const T *a, size_t lda, const T *b, size_t ldb, T beta, T *c,
+                     size_t ldc, bool trans_b, bool lower_only) {
+    if (m == 0 || n == 0) return RLG_OK;
+    RLG_TRY(check_dims(m, n, k));
+    const int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
+    dim3 grid(tiles_m * tiles_n), block(GTHREADS);
+    const double flops = (lower_only ? 1.0 : 2.0) * (double)m * (double)n * (double)k;
+    static const int cls_id = prof_class(cls);
+    ProfScope prof_scope(cls_id, s, flops);
+#define RLG_GEMM_LAUNCH(TB, LO)                                                              \
+    hipLaunchKernelGGL((k_gemm<T, TB, LO>), grid, block, 0, s, (int)m, (int)n, (int)k, alpha, \
+                       a, (long)lda, b, (long)ldb, beta, c, (long)ldc, tiles_m, tiles_n)
+    if (trans_b) {
+        if (lower_only) RLG_GEMM_LAUNCH(true, true); else RLG_GEMM_LAUNCH(true, false);
+    } else {
+        if (lower_only) RLG_GEMM_LAUNCH(false, true); else RLG_GEMM_LAUNCH(false, false);
+    }
+#undef RLG_GEMM_LAUNCH
+    RLG_LAUNCHED();
+    return RLG_OK;
+}
+
+int gemm_f64(hipStream_t s, size_t m, size_t n, size_t k, double alpha, const double *a,
+             size_t lda, const double *b, size_t ldb, double beta, double *c, size_t ldc,
+             bool trans_b, bool lower_only) {
+    return gemm_impl<double>(s, "gemm_f64", m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                             trans_b, lower_only);
+}
+
+int gemm_f32(hipStream_t s, size_t m, size_t n, size_t k, float alpha, const float *a,
+             size_t lda, const float *b, size_t ldb, float beta, float *c, size_t ldc,
+             bool trans_b, bool lower_only) {
+    return gemm_impl<float>(s, "gemm_f32", m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                            trans_b, lower_only);
+}
+
+int gemm_lowp(hipStream_t s, int prec, size_t m, size_t n, size_t k, float alpha,
+              const float *a, size_t lda, const float *b, size_t ldb, float beta, float *c,
+              size_t ldc, bool trans_b, bool lower_only) {
+    if (m == 0 || n == 0) return RLG_OK;
+    RLG_TRY(check_dims(m, n, k));
+    const int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
+    dim3 grid(tiles_m * tiles_n), block(GTHREADS);
+#define RLG_LOWP_LAUNCH(H, TB, LO)                                                           \
+    hipLaunchKernelGGL((k_gemm_lowp<H, TB, LO>), grid, block, 0, s, (int)m, (int)n, (int)k,  \
+                       alpha, a, (long)lda, b, (long)ldb, beta, c, (long)ldc, tiles_m,       \
+                       tiles_n)
+    const double flops = (lower_only ? 1.0 : 2.0) * (double)m * (double)n * (double)k;
+    if (prec == RLG_PREC_F16) {
+        RLG_PROF_W("gemm_f16", s, flops);
+        if (trans_b) {
+            if (lower_only) RLG_LOWP_LAUNCH(_Float16, true, true); else RLG_LOWP_LAUNCH(_Float16, true, false);
+        } else {
+            if (lower_only) RLG_LOWP_LAUNCH(_Float16, false, true); else RLG_LOWP_LAUNCH(_Float16, false, false);
+        }
+    } else if (prec == RLG_PREC_BF16) {
+        RLG_PROF_W("gemm_bf16", s, flops);
+        if (trans_b) {
+            if (lower_only) RLG_LOWP_LAUNCH(__bf16, true, true); else RLG_LOWP_LAUNCH(__bf16, true, false);
+        } else {
+            if (lower_only) RLG_LOWP_LAUNCH(__bf16, false, true); else RLG_LOWP_LAUNCH(__bf16, false, false);
+        }
+    } else {
+        set_error("gemm_lowp: precision %d is not FP16/BF16", prec);
+        return RLG_E_INVALID_ARG;
+    }
+#undef RLG_LOWP_LAUNCH
+    RLG_LAUNCHED();
+    return RLG_OK;
+}
+
+// ---------------------------------------------------------------- GEMV
+
+// y = A x, one wave per row, FP64.
+__global__ __launch_bounds__(256) void k_gemv_f64(int m, int n, const double *__restrict__ a,
+                                                  long lda, const double *__restrict__ x,
+                                                  double *__restrict__ y) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= m) return;
+    const double *r = a + (long)row * lda;
+    double s = 0.0;
+    for (int j = lane; j < n; j += 64) s += r[j] * x[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) y[row] = s;
+}
+
+int gemv_f64(hipStream_t s, size_t m, size_t n, const double *a, size_t lda, const double *x,
+             double *y) {
+    if (m == 0) return RLG_OK;
+    RLG_TRY(check_dims(m, n, 1));
+    RLG_PROF("gemv_f64", s);
+    hipLaunchKernelGGL(k_gemv_f64, dim3(ceil_div(m, 4)), dim3(256), 0, s, (int)m, (int)n, a,
+                       (long)lda, x, y);
+    RLG_LAUNCHED();
+    return RLG_OK;
+}
+
+}  // namespace rlg

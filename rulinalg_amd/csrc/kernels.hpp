@@ -1,0 +1,110 @@
+// Internal launchers shared between the translation units of librlgpu.
+// All matrices are row-major with explicit row strides; all launches are
+// asynchronous on the given stream.
+#pragma once
+
+#include "common.hpp"
+
+namespace rlg {
+
+// C = alpha * op(A) * op(B) + beta * C on the MFMA pipes.
+//   A is m x k; B is k x n, or n x k when trans_b (then op(B) = B^T).
+//   lower_only: C is square (m == n) and only tiles touching the lower
+//   triangle are computed (the SYRK shape of the Cholesky update).
+// beta == 0 never reads C.
+int gemm_f64(hipStream_t s, size_t m, size_t n, size_t k, double alpha,
+             const double *a, size_t lda, const double *b, size_t ldb,
+             double beta, double *c, size_t ldc, bool trans_b = false,
+             bool lower_only = false);
+int gemm_f32(hipStream_t s, size_t m, size_t n, size_t k, float alpha,
+             const float *a, size_t lda, const float *b, size_t ldb,
+             float beta, float *c, size_t ldc, bool trans_b = false,
+             bool lower_only = false);
+// FP32 operands rounded to FP16 (prec = RLG_PREC_F16) or BF16
+// (RLG_PREC_BF16) on their way into LDS, products accumulated in FP32.
+int gemm_lowp(hipStream_t s, int prec, size_t m, size_t n, size_t k,
+              float alpha, const float *a, size_t lda, const float *b,
+              size_t ldb, float beta, float *c, size_t ldc,
+              bool trans_b = false, bool lower_only = false);
+
+template <class T>
+inline int gemm_t(hipStream_t s, size_t m, size_t n, size_t k, T alpha,
+                  const T *a, size_t lda, const T *b, size_t ldb, T beta, T *c,
+                  size_t ldc, bool trans_b = false, bool lower_only = false);
+template <>
+inline int gemm_t<double>(hipStream_t s, size_t m, size_t n, size_t k,
+                          double alpha, const double *a, size_t lda,
+                          const double *b, size_t ldb, double beta, double *c,
+                          size_t ldc, bool tb, bool lo) {
+    return gemm_f64(s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tb, lo);
+}
+template <>
+inline int gemm_t<float>(hipStream_t s, size_t m, size_t n, size_t k,
+                         float alpha, const float *a, size_t lda,
+                         const float *b, size_t ldb, float beta, float *c,
+                         size_t ldc, bool tb, bool lo) {
+    return gemm_f32(s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tb, lo);
+}
+
+// y = A x (FP64, one row per wave), A m x n.
+int gemv_f64(hipStream_t s, size_t m, size_t n, const double *a, size_t lda,
+             const double *x, double *y);
+
+// ------------------------------------------------------------------ LU
+// Right-looking blocked LU with partial pivoting of the n x n matrix `a`
+// in place.  ipiv[k] receives the (0-based, absolute) row swapped with row
+// k at step k.  *info (device) receives the first step whose pivot was
+// below epsilon in magnitude, or n if none.  Also uses `work`, a device
+// buffer of lu_work_bytes(n) bytes.
+size_t lu_work_bytes(size_t n, size_t elem);
+template <class T>
+int lu_factor(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv,
+              int64_t *info, void *work);
+// Mixed-precision variant of the FP32 factorization whose trailing updates
+// run on FP16/BF16 MFMA (prec = RLG_PREC_F16 / RLG_PREC_BF16).
+int lu_factor_lowp(hipStream_t s, int prec, size_t n, float *a, size_t lda,
+                   int64_t *ipiv, int64_t *info, void *work);
+
+// Device flag raised by a panel barrier that gave up waiting (see lu.hip);
+// nonzero after lu_factor means the factorization is invalid.
+int *lu_timeout_flag(void *work, size_t n, size_t elem);
+
+// Row-permutation map from the step pivots (host): raw[k] = the original
+// row now at position k, i.e. the product of the swaps before the
+// reference's final inverse (lu.rs:186-193).
+void perm_from_ipiv(size_t n, const int64_t *ipiv, int64_t *raw);
+
+// X = U^-1 L^-1 P B for nrhs vectors (vector r at b + r*ldb); x may alias b.
+// `tmp` holds n * nrhs elements.
+template <class T>
+int lu_solve(hipStream_t s, size_t n, const T *lu, size_t lda,
+             const int64_t *raw, size_t nrhs, const T *b, size_t ldb, T *x,
+             size_t ldx, T *tmp);
+
+// ------------------------------------------------------------------ Cholesky
+size_t chol_work_bytes(size_t n, size_t elem);
+template <class T>
+int chol_factor(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info,
+                void *work);
+int chol_factor_lowp(hipStream_t s, int prec, size_t n, float *a, size_t lda,
+                     int64_t *info, void *work);
+template <class T>
+int chol_solve(hipStream_t s, size_t n, const T *l, size_t lda, size_t nrhs,
+               const T *b, size_t ldb, T *x, size_t ldx);
+
+// ------------------------------------------------------------------ triangular
+// In-place solves on nrhs vectors (vector r at x + r*ldx):
+//   lower:  L y = x   (unit: diagonal taken as 1)
+//   upper:  U y = x
+//   lower_t: L^T y = x
+template <class T>
+int trsv_lower(hipStream_t s, size_t n, const T *l, size_t lda, bool unit,
+               size_t nrhs, T *x, size_t ldx);
+template <class T>
+int trsv_upper(hipStream_t s, size_t n, const T *u, size_t lda, size_t nrhs,
+               T *x, size_t ldx);
+template <class T>
+int trsv_lower_t(hipStream_t s, size_t n, const T *l, size_t lda, size_t nrhs,
+                 T *x, size_t ldx);
+
+}  // namespace rlg

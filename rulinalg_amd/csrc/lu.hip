@@ -1,0 +1,516 @@
+// Right-looking blocked LU with partial pivoting (PartialPivLu::decompose,
+// src/matrix/decomposition/lu.rs:163-194) and its solve (lu.rs:231-244).
+//
+// Blocking: outer blocks of ONB = 256 columns drive the trailing Schur
+// update as one MFMA GEMM of depth 256; inside an outer block, inner panels
+// of PNB = 64 columns are factored by a cooperative kernel and applied to
+// the rest of the outer block by a 64-row triangular solve and a depth-64
+// GEMM.
+//
+// The panel kernel keeps the whole (n - k) x 64 panel resident in LDS,
+// spread over G workgroups of PR rows each.  Every column costs one
+// grid-wide barrier: each workgroup publishes its local pivot candidate
+// (|value|, row, and the full 64-wide row) and the content of the current
+// diagonal row; after the barrier every workgroup reduces the G candidates
+// to the same winner, performs its part of the row swap, and eliminates its
+// rows.  The pivot rule is the reference's: the first row of maximal
+// magnitude at or below the diagonal (strict `>` scan, lu.rs:170-178); the
+// multiplier is a division and the update a separately rounded multiply and
+// subtract (lu.rs:603-616), so a matrix no wider than one panel is factored
+// bit-identically to the reference.
+#include <climits>
+
+#include "kernels.hpp"
+
+namespace rlg {
+
+constexpr int PNB = 64;    // inner panel width
+constexpr int PR = 128;    // panel rows per workgroup (LDS-resident)
+constexpr int ONB = 256;   // outer block: depth of the trailing GEMM
+constexpr int PTHREADS = 256;
+
+template <class T> __device__ __forceinline__ T eps_of();
+template <> __device__ __forceinline__ double eps_of<double>() { return 2.220446049250313e-16; }
+template <> __device__ __forceinline__ float eps_of<float>() { return 1.1920929e-7f; }
+
+// Workspace layout (bytes): barrier counters, candidate slots, swap map.
+struct LuWork {
+    unsigned *bar;     // one counter per inner panel
+    void *cval;        // [2][GMAX] T
+    int *crow;         // [2][GMAX]
+    void *cdat;        // [2][GMAX][PNB] T
+    void *rowj;        // [2][PNB] T
+    int *map;          // [1 + 2 * 2 * ONB]: count, dst[], src[]
+    int *timeout;      // barrier timeout flag
+};
+constexpr int GMAX = 1024;
+
+static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t lu_work_bytes(size_t n, size_t elem) {
+    size_t panels = (n + PNB - 1) / PNB + 1;
+    return align_up(panels * 4) + align_up(2 * GMAX * elem) + align_up(2 * GMAX * 4) +
+           align_up(2 * GMAX * PNB * elem) + align_up(2 * PNB * elem) +
+           align_up((1 + 4 * ONB) * 4) + align_up(4);
+}
+
+static LuWork carve(void *work, size_t n, size_t elem) {
+    char *p = static_cast<char *>(work);
+    LuWork w;
+    size_t panels = (n + PNB - 1) / PNB + 1;
+    w.bar = reinterpret_cast<unsigned *>(p); p += align_up(panels * 4);
+    w.cval = p; p += align_up(2 * GMAX * elem);
+    w.crow = reinterpret_cast<int *>(p); p += align_up(2 * GMAX * 4);
+    w.cdat = p; p += align_up(2 * GMAX * PNB * elem);
+    w.rowj = p; p += align_up(2 * PNB * elem);
+    w.map = reinterpret_cast<int *>(p); p += align_up((1 + 4 * ONB) * 4);
+    w.timeout = reinterpret_cast<int *>(p);
+    return w;
+}
+
+int *lu_timeout_flag(void *work, size_t n, size_t elem) { return carve(work, n, elem).timeout; }
+
+void perm_from_ipiv(size_t n, const int64_t *ipiv, int64_t *raw) {
+    for (size_t i = 0; i < n; ++i) raw[i] = (int64_t)i;
+    for (size_t k = 0; k < n; ++k) {
+        const size_t p = (size_t)ipiv[k];
+        if (p != k) { int64_t t = raw[k]; raw[k] = raw[p]; raw[p] = t; }
+    }
+}
+
+// ---------------------------------------------------------------- panel
+
+// Candidate order: larger magnitude first; equal magnitudes go to the lower
+// row index, which reproduces the reference's first-maximum scan.
+template <class T>
+__device__ __forceinline__ bool better(T v2, int i2, T v1, int i1) {
+    if (i2 == INT_MAX) return false;
+    if (i1 == INT_MAX) return true;
+    if (v2 > v1) return true;
+    if (v1 > v2) return false;
+    return i2 < i1;
+}
+
+template <class T>
+__device__ __forceinline__ void wave_argmax(T &v, int &i) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        T ov = __shfl_xor(v, off, 64);
+        int oi = __shfl_xor(i, off, 64);
+        if (better(ov, oi, v, i)) { v = ov; i = oi; }
+    }
+}
+
+// Spin until *bar reaches target.  Bounded: after 2^22 polls the shared
+// timeout flag is raised; every waiter also leaves as soon as it sees the
+// flag, so a barrier that cannot complete ends the kernel (with an error the
+// host reports) instead of hanging it.
+__device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int *timeout) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            if (++spins > (1u << 22)) {
+                __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Factor the panel A[k:k+m, k:k+w] (w <= PNB) in place.
+template <class T>
+__global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long lda, int k, int m,
+                                                       int w, int64_t *__restrict__ ipiv,
+                                                       int64_t *__restrict__ info, int n,
+                                                       unsigned *bar, T *cval, int *crow,
+                                                       T *cdat, T *rowj, int *timeout) {
+    __shared__ T tile[PR][PNB + 1];
+    __shared__ T prow[PNB];
+    __shared__ T red_v[PTHREADS / 64];
+    __shared__ int red_i[PTHREADS / 64];
+    __shared__ int s_piv;
+
+    const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
+    const int r0 = g * PR;
+    const int rows = min(PR, m - r0);
+    const int lane = tid & 63, wave = tid >> 6;
+
+    for (int e = tid; e < rows * w; e += PTHREADS) {
+        const int r = e / w, c = e - r * w;
+        tile[r][c] = A[(long)(k + r0 + r) * lda + k + c];
+    }
+    __syncthreads();
+
+    for (int j = 0; j < w; ++j) {
+        const int par = j & 1;
+        // 1. local candidate: first maximum of |tile[r][j]| over owned rows >= j
+        T bv = T(0);
+        int bi = INT_MAX;
+        for (int r = tid; r < rows; r += PTHREADS) {
+            const int gr = r0 + r;
+            if (gr >= j) {
+                T v = fabs(tile[r][j]);
+                if (better(v, gr, bv, bi)) { bv = v; bi = gr; }
+            }
+        }
+        wave_argmax(bv, bi);
+        if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
+        __syncthreads();
+        if (tid == 0) {
+            for (int q = 1; q < PTHREADS / 64; ++q)
+                if (better(red_v[q], red_i[q], bv, bi)) { bv = red_v[q]; bi = red_i[q]; }
+            red_v[0] = bv;
+            red_i[0] = bi;
+        }
+        __syncthreads();
+        const int li = red_i[0];
+        const bool own_j = (j >= r0 && j < r0 + rows);
+
+        int p;
+        if (G > 1) {
+            // 2. publish candidate row and the diagonal row, then meet
+            T *my_cdat = cdat + ((long)par * G + g) * PNB;
+            if (tid == 0) { cval[par * G + g] = red_v[0]; crow[par * G + g] = li; }
+            if (li != INT_MAX && tid < w) my_cdat[tid] = tile[li - r0][tid];
+            if (own_j && tid < w) rowj[par * PNB + tid] = tile[j - r0][tid];
+            grid_barrier(bar, (unsigned)(G * (j + 1)), timeout);
+            // 3. every workgroup reduces all candidates to the same winner
+            if (wave == 0) {
+                T v = T(0);
+                int i = INT_MAX, gw = 0;
+                for (int q = lane; q < G; q += 64) {
+                    T qv = cval[par * G + q];
+                    int qi = crow[par * G + q];
+                    if (better(qv, qi, v, i)) { v = qv; i = qi; gw = q; }
+                }
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    T ov = __shfl_xor(v, off, 64);
+                    int oi = __shfl_xor(i, off, 64);
+                    int og = __shfl_xor(gw, off, 64);
+                    if (better(ov, oi, v, i)) { v = ov; i = oi; gw = og; }
+                }
+                if (lane == 0) s_piv = i;
+                // winner's row data
+                if (lane < w) prow[lane] = cdat[((long)par * G + gw) * PNB + lane];
+            }
+            __syncthreads();
+            p = s_piv;
+            // 4. swap rows j and p in the owning tiles (old row j comes from
+            //    the published copy when it lives in another workgroup)
+            if (p != j && tid < w) {
+                const T old_j = own_j ? tile[j - r0][tid] : rowj[par * PNB + tid];
+                if (own_j) tile[j - r0][tid] = prow[tid];
+                if (p >= r0 && p < r0 + rows) tile[p - r0][tid] = old_j;
+            }
+        } else {
+            p = li;
+            if (tid < w) {
+                const T vp = tile[p][tid];
+                const T vj = tile[j][tid];
+                prow[tid] = vp;
+                if (p != j) { tile[j][tid] = vp; tile[p][tid] = vj; }
+            }
+        }
+        __syncthreads();
+        if (g == 0 && tid == 0) {
+            ipiv[k + j] = (int64_t)(k + p);
+            if (fabs(prow[j]) < eps_of<T>() && *info == (int64_t)n) *info = k + j;
+        }
+
+        // 5. eliminate: mult = a[i][j] / piv; a[i][c] = a[i][c] - mult * piv_row[c]
+        {
+#pragma clang fp contract(off)
+            const T piv = prow[j];
+            const int r = tid & (PR - 1), h = tid / PR;
+            const bool act = r < rows && r0 + r > j;
+            T mult = T(0);
+            if (act) mult = tile[r][j] / piv;
+            __syncthreads();
+            if (act) {
+                if (h == 0) tile[r][j] = mult;
+                for (int c = j + 1 + h; c < w; c += PTHREADS / PR) tile[r][c] = tile[r][c] - mult * prow[c];
+            }
+            __syncthreads();
+        }
+    }
+
+    for (int e = tid; e < rows * w; e += PTHREADS) {
+        const int r = e / w, c = e - r * w;
+        A[(long)(k + r0 + r) * lda + k + c] = tile[r][c];
+    }
+}
+
+// ---------------------------------------------------------------- row swaps
+
+// Net effect of the swaps ipiv[k0 .. k0+cnt) as a list of (dst, src) rows:
+// after the swaps, row dst holds what row src held before.  One wave; the
+// rows outside [k0, k0+cnt) that get touched are tracked in a small table
+// searched with a ballot.
+__global__ __launch_bounds__(64) void k_swap_map(const int64_t *__restrict__ ipiv, int k0, int cnt,
+                                                 int *__restrict__ map) {
+    __shared__ int content[ONB];          // content[r - k0] for r in [k0, k0+cnt)
+    __shared__ int ext_row[ONB], ext_src[ONB];
+    __shared__ int n_ext;
+    const int lane = threadIdx.x;
+    for (int i = lane; i < cnt; i += 64) content[i] = k0 + i;
+    if (lane == 0) n_ext = 0;
+    __syncthreads();
+    for (int s = 0; s < cnt; ++s) {
+        const int r = k0 + s;
+        const int p = (int)ipiv[k0 + s];
+        if (p == r) continue;
+        if (p < k0 + cnt) {
+            if (lane == 0) { int t = content[s]; content[s] = content[p - k0]; content[p - k0] = t; }
+        } else {
+            // find p among the external rows
+            int found = -1;
+            const int ne = n_ext;
+            for (int base = 0; base < ne; base += 64) {
+                const int q = base + lane;
+                const bool hit = q < ne && ext_row[q] == p;
+                const unsigned long long bal = __ballot(hit);
+                if (bal) { found = base + __ffsll((long long)bal) - 1; break; }
+            }
+            if (lane == 0) {
+                int idx = found;
+                if (idx < 0) { idx = ne; ext_row[idx] = p; ext_src[idx] = p; n_ext = ne + 1; }
+                int t = content[s]; content[s] = ext_src[idx]; ext_src[idx] = t;
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        int c = 0;
+        for (int i = 0; i < cnt; ++i)
+            if (content[i] != k0 + i) { map[1 + c] = k0 + i; map[1 + 2 * ONB + c] = content[i]; ++c; }
+        for (int i = 0; i < n_ext; ++i)
+            if (ext_src[i] != ext_row[i]) { map[1 + c] = ext_row[i]; map[1 + 2 * ONB + c] = ext_src[i]; ++c; }
+        map[0] = c;
+    }
+}
+
+// Applies a swap map to columns [c0, c1) and [c2, c3).  Each workgroup owns
+// LCOLS columns: it reads every source row of the map into LDS, then writes
+// the destinations, so the in-place permutation needs no ordering.
+constexpr int LCOLS = 32;
+template <class T>
+__global__ __launch_bounds__(256) void k_laswp(T *__restrict__ A, long lda, const int *__restrict__ map,
+                                               int c0, int c1, int c2, int c3) {
+    __shared__ T stage[2 * ONB][LCOLS];
+    __shared__ int dst[2 * ONB], src[2 * ONB];
+    const int cnt = map[0];
+    if (cnt == 0) return;
+    for (int i = threadIdx.x; i < cnt; i += 256) { dst[i] = map[1 + i]; src[i] = map[1 + 2 * ONB + i]; }
+    // column block -> absolute columns
+    const int w1 = c1 - c0;
+    const int cb = blockIdx.x * LCOLS;
+    __syncthreads();
+    const int cl = threadIdx.x % LCOLS, rg = threadIdx.x / LCOLS;  // 8 row groups
+    const int lc = cb + cl;
+    const int col = lc < w1 ? c0 + lc : c2 + (lc - w1);
+    const bool okc = lc < w1 + (c3 - c2);
+    for (int i = rg; i < cnt; i += 256 / LCOLS)
+        if (okc) stage[i][cl] = A[(long)src[i] * lda + col];
+    __syncthreads();
+    for (int i = rg; i < cnt; i += 256 / LCOLS)
+        if (okc) A[(long)dst[i] * lda + col] = stage[i][cl];
+}
+
+// ---------------------------------------------------------------- TRSM
+
+// B := L^-1 B for the unit-lower jb x jb block L = A[j0:j0+jb, j0:j0+jb]
+// (jb <= 64) and B = A[j0:j0+jb, c0:c1), in place, column sweep:
+// for each step c, b[r] = b[r] - l[r][c] * b[c] for r > c.  This is the
+// order in which gaussian_elimination (lu.rs:603-616) updates the rows of
+// the block, with the same separately rounded multiply and subtract.
+template <class T>
+__global__ __launch_bounds__(256) void k_trsm_llu(T *__restrict__ A, long lda, int j0, int jb, int c0,
+                                                  int c1) {
+    __shared__ T L[PNB][PNB + 1];
+    __shared__ T X[PNB][65];
+    const int tid = threadIdx.x;
+    const int cb = c0 + blockIdx.x * 64;
+    const int nc = min(64, c1 - cb);
+    for (int e = tid; e < jb * jb; e += 256) {
+        const int r = e / jb, c = e - r * jb;
+        L[r][c] = A[(long)(j0 + r) * lda + j0 + c];
+    }
+    for (int e = tid; e < jb * 64; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        X[r][c] = c < nc ? A[(long)(j0 + r) * lda + cb + c] : T(0);
+    }
+    __syncthreads();
+    const int cj = tid & 63, rq = tid >> 6;
+    {
+#pragma clang fp contract(off)
+        for (int c = 0; c < jb - 1; ++c) {
+            const T xc = X[c][cj];
+            for (int r = c + 1 + rq; r < jb; r += 4) X[r][cj] = X[r][cj] - L[r][c] * xc;
+            __syncthreads();
+        }
+    }
+    for (int e = tid; e < jb * 64; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        if (c < nc) A[(long)(j0 + r) * lda + cb + c] = X[r][c];
+    }
+}
+
+// ---------------------------------------------------------------- driver
+
+template <class T>
+static int panel(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, int64_t *ipiv,
+                 int64_t *info, size_t n, const LuWork &lw, unsigned *bar) {
+    const unsigned G = ceil_div(m, PR);
+    if (G > GMAX) {
+        set_error("LU panel of %zu rows needs %u workgroups (max %d)", m, G, GMAX);
+        return RLG_E_INVALID_ARG;
+    }
+    RLG_PROF("lu_panel", s);
+    hipLaunchKernelGGL(k_lu_panel<T>, dim3(G), dim3(PTHREADS), 0, s, a, (long)lda, (int)k, (int)m,
+                       (int)w, ipiv, info, (int)n, bar, static_cast<T *>(lw.cval), lw.crow,
+                       static_cast<T *>(lw.cdat), static_cast<T *>(lw.rowj), lw.timeout);
+    RLG_LAUNCHED();
+    return RLG_OK;
+}
+
+template <class T>
+static int laswp(hipStream_t s, T *a, size_t lda, const int64_t *ipiv, size_t k0, size_t cnt,
+                 size_t c0, size_t c1, size_t c2, size_t c3, const LuWork &lw) {
+    const size_t ncols = (c1 - c0) + (c3 - c2);
+    if (ncols == 0 || cnt == 0) return RLG_OK;
+    RLG_PROF("laswp", s);
+    hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, s, ipiv, (int)k0, (int)cnt, lw.map);
+    RLG_LAUNCHED();
+    hipLaunchKernelGGL(k_laswp<T>, dim3(ceil_div(ncols, LCOLS)), dim3(256), 0, s, a, (long)lda,
+                       lw.map, (int)c0, (int)c1, (int)c2, (int)c3);
+    RLG_LAUNCHED();
+    return RLG_OK;
+}
+
+template <class T>
+static int trsm_llu(hipStream_t s, T *a, size_t lda, size_t j0, size_t jb, size_t c0, size_t c1) {
+    if (c1 <= c0 || jb <= 1) return RLG_OK;
+    RLG_PROF("trsm", s);
+    hipLaunchKernelGGL(k_trsm_llu<T>, dim3(ceil_div(c1 - c0, 64)), dim3(256), 0, s, a, (long)lda,
+                       (int)j0, (int)jb, (int)c0, (int)c1);
+    RLG_LAUNCHED();
+    return RLG_OK;
+}
+
+template <class T>
+using UpdateFn = int (*)(hipStream_t, size_t, size_t, size_t, const T *, size_t, const T *, size_t,
+                         T *, size_t);
+
+// C -= A * B in working precision.
+template <class T>
+static int update_native(hipStream_t s, size_t m, size_t n, size_t k, const T *a, size_t lda,
+                         const T *b, size_t ldb, T *c, size_t ldc) {
+    return gemm_t<T>(s, m, n, k, T(-1), a, lda, b, ldb, T(1), c, ldc);
+}
+
+static thread_local int g_lowp_prec = RLG_PREC_F16;  // set by lu_factor_lowp for its duration
+static int update_lowp(hipStream_t s, size_t m, size_t n, size_t k, const float *a, size_t lda,
+                       const float *b, size_t ldb, float *c, size_t ldc) {
+    return gemm_lowp(s, g_lowp_prec, m, n, k, -1.f, a, lda, b, ldb, 1.f, c, ldc);
+}
+
+template <class T>
+static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv, int64_t *info,
+                          void *work, UpdateFn<T> big_update) {
+    LuWork lw = carve(work, n, sizeof(T));
+    const size_t panels = (n + PNB - 1) / PNB + 1;
+    RLG_TRY(fill_u32(lw.bar, panels, 0u, s));
+    RLG_TRY(fill_u32(reinterpret_cast<uint32_t *>(lw.timeout), 1, 0u, s));
+    RLG_TRY(fill_i64(info, 1, (int64_t)n, s));
+    size_t pidx = 0;
+    for (size_t k0 = 0; k0 < n; k0 += ONB) {
+        const size_t kb = n - k0 < (size_t)ONB ? n - k0 : ONB;
+        for (size_t j0 = k0; j0 < k0 + kb; j0 += PNB) {
+            const size_t jb = k0 + kb - j0 < (size_t)PNB ? k0 + kb - j0 : PNB;
+            RLG_TRY(panel<T>(s, a, lda, j0, n - j0, jb, ipiv, info, n, lw, lw.bar + pidx));
+            ++pidx;
+            // inner swaps on the other columns of the outer block
+            RLG_TRY(laswp<T>(s, a, lda, ipiv, j0, jb, k0, j0, j0 + jb, k0 + kb, lw));
+            if (j0 + jb < k0 + kb) {
+                RLG_TRY(trsm_llu<T>(s, a, lda, j0, jb, j0 + jb, k0 + kb));
+                RLG_TRY(update_native<T>(s, n - j0 - jb, k0 + kb - j0 - jb, jb,
+                                         a + (j0 + jb) * lda + j0, lda, a + j0 * lda + j0 + jb, lda,
+                                         a + (j0 + jb) * lda + j0 + jb, lda));
+            }
+        }
+        // outer swaps on everything left and right of the outer block
+        RLG_TRY(laswp<T>(s, a, lda, ipiv, k0, kb, 0, k0, k0 + kb, n, lw));
+        if (k0 + kb < n) {
+            const size_t c0 = k0 + kb;
+            for (size_t j0 = k0; j0 < k0 + kb; j0 += PNB) {
+                const size_t jb = k0 + kb - j0 < (size_t)PNB ? k0 + kb - j0 : PNB;
+                RLG_TRY(trsm_llu<T>(s, a, lda, j0, jb, c0, n));
+                if (j0 + jb < k0 + kb)
+                    RLG_TRY(update_native<T>(s, k0 + kb - j0 - jb, n - c0, jb, a + (j0 + jb) * lda + j0,
+                                             lda, a + j0 * lda + c0, lda, a + (j0 + jb) * lda + c0, lda));
+            }
+            RLG_TRY(big_update(s, n - c0, n - c0, kb, a + c0 * lda + k0, lda, a + k0 * lda + c0, lda,
+                               a + c0 * lda + c0, lda));
+        }
+    }
+    return RLG_OK;
+}
+
+template <class T>
+int lu_factor(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv, int64_t *info, void *work) {
+    return lu_factor_impl<T>(s, n, a, lda, ipiv, info, work, &update_native<T>);
+}
+template int lu_factor<double>(hipStream_t, size_t, double *, size_t, int64_t *, int64_t *, void *);
+template int lu_factor<float>(hipStream_t, size_t, float *, size_t, int64_t *, int64_t *, void *);
+
+int lu_factor_lowp(hipStream_t s, int prec, size_t n, float *a, size_t lda, int64_t *ipiv,
+                   int64_t *info, void *work) {
+    g_lowp_prec = prec;
+    return lu_factor_impl<float>(s, n, a, lda, ipiv, info, work, &update_lowp);
+}
+
+// ---------------------------------------------------------------- solve
+
+// tmp[r][k] = b[r][raw[k]]: y = P b with (P b)[k] = b[raw[k]], which is the
+// reference's permute_vector_into_buffer (permutation_matrix.rs:369-382)
+// applied with the stored inverse permutation.
+template <class T>
+__global__ void k_gather_rows(size_t n, size_t nrhs, const int64_t *__restrict__ raw,
+                              const T *__restrict__ b, size_t ldb, T *__restrict__ y, size_t ldy) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t r = blockIdx.y;
+    if (i < n && r < nrhs) y[r * ldy + i] = b[r * ldb + raw[i]];
+}
+
+template <class T>
+int lu_solve(hipStream_t s, size_t n, const T *lu, size_t lda, const int64_t *raw, size_t nrhs,
+             const T *b, size_t ldb, T *x, size_t ldx, T *tmp) {
+    if (n == 0 || nrhs == 0) return RLG_OK;
+    {
+        RLG_PROF("gather", s);
+        hipLaunchKernelGGL(k_gather_rows<T>, dim3(ceil_div(n, 256), (unsigned)nrhs), dim3(256), 0, s, n,
+                           nrhs, raw, b, ldb, tmp, n);
+        RLG_LAUNCHED();
+    }
+    RLG_TRY(trsv_lower<T>(s, n, lu, lda, true, nrhs, tmp, n));
+    RLG_TRY(trsv_upper<T>(s, n, lu, lda, nrhs, tmp, n));
+    for (size_t r = 0; r < nrhs; ++r)
+        RLG_HIP(hipMemcpyAsync(x + r * ldx, tmp + r * n, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    return RLG_OK;
+}
+template int lu_solve<double>(hipStream_t, size_t, const double *, size_t, const int64_t *, size_t,
+                              const double *, size_t, double *, size_t, double *);
+template int lu_solve<float>(hipStream_t, size_t, const float *, size_t, const int64_t *, size_t,
+                             const float *, size_t, float *, size_t, float *);
+
+}  // namespace rlg

@@ -1,0 +1,210 @@
+"""GPU: batched uneven systems and the continuous-batching server, against the
+CPU oracle.
+
+Tolerances: batched plain FP64 results follow the same rules as the single
+system path (pivots identical where the oracle's margins are clear; solutions
+with normwise backward error <= 1e-13 and forward difference
+<= max(1e-13, 16 n kappa u)); mixed results must reach backward error <= 1e-13
+(directly or through the FP64 fallback).  The server's results must be
+bit-identical alone and in a mix.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from rulinalg_amd import _lib, batch, workloads as wl
+
+pytestmark = pytest.mark.gpu
+U = np.finfo(np.float64).eps
+
+
+@pytest.fixture(scope="module", autouse=True)
+def lib():
+    lib = _lib.load()
+    assert lib.rlg_set_device(0) == 0, _lib.last_error()
+    return lib
+
+
+def eta(a, x, b):
+    return np.abs(b - a @ x).max() / (np.abs(a).sum(1).max() * np.abs(x).max() + np.abs(b).max())
+
+
+def make_systems(count, seed, nmax=512):
+    r = wl.rng(seed)
+    out = []
+    for i in range(count):
+        n = int(r.integers(1, nmax + 1))
+        nr = int(r.integers(1, 9))
+        kind = batch.KIND_LU if i % 2 == 0 else batch.KIND_CHOL
+        if kind == batch.KIND_LU:
+            a = r.standard_normal((n, n))
+        else:
+            g = r.standard_normal((n, n))
+            a = g @ g.T / n + np.eye(n)
+        b = r.standard_normal((nr, n))
+        out.append((kind, a, b))
+    return out
+
+
+def check_against_oracle(kind, a, b, x, factor=None, ipiv=None):
+    n = a.shape[0]
+    if kind == batch.KIND_LU:
+        lu_or, perm_or, ipiv_or, margin = orc.lu_decompose(a, with_margin=True)
+        if ipiv is not None:
+            clear = margin > 1e-9 * np.abs(a).max()
+            fu = np.argmin(clear) if not np.all(clear) else n
+            assert np.array_equal(ipiv[:fu], ipiv_or[:fu])
+        for i in range(b.shape[0]):
+            xo = orc.lu_solve(lu_or, perm_or, b[i])
+            assert eta(a, x[i], b[i]) <= 1e-13
+            k = np.linalg.cond(a, np.inf)
+            assert np.abs(x[i] - xo).max() / np.abs(xo).max() <= max(1e-13, 16 * n * k * U)
+    else:
+        w = orc.chol_decompose(a)
+        if factor is not None:
+            l_or = orc.chol_unpack(w)
+            assert np.abs(np.tril(factor) - l_or).max() / np.abs(l_or).max() <= 1e-10
+        for i in range(b.shape[0]):
+            xo = orc.chol_solve(w, b[i])
+            assert eta(a, x[i], b[i]) <= 1e-13
+            k = np.linalg.cond(a, np.inf)
+            assert np.abs(x[i] - xo).max() / np.abs(xo).max() <= max(1e-13, 16 * n * k * U)
+
+
+def test_batch_small_bit_exact():
+    """Orders <= 16 fit one panel (NB = 16): factors and pivots bit-identical."""
+    r = wl.rng(5)
+    systems = []
+    for n in [1, 2, 3, 8, 13, 16]:
+        systems.append((batch.KIND_LU, r.standard_normal((n, n)), r.standard_normal((2, n))))
+    out = batch.solve_batch(systems)
+    for (kind, a, b), f, ip, st in zip(systems, out["factor"], out["ipiv"], out["status"]):
+        assert st == 0
+        lu_or, perm_or, ipiv_or = orc.lu_decompose(a)
+        assert np.array_equal(ip, ipiv_or)
+        assert np.array_equal(f, lu_or)
+
+
+def test_batch_plain_vs_oracle():
+    systems = make_systems(40, 11)
+    out = batch.solve_batch(systems)
+    for (kind, a, b), x, f, ip, st in zip(systems, out["x"], out["factor"], out["ipiv"], out["status"]):
+        assert st == 0
+        check_against_oracle(kind, a, b, x, f if kind == batch.KIND_CHOL else None,
+                             ip if kind == batch.KIND_LU else None)
+
+
+def test_batch_failures_reported_per_item():
+    r = wl.rng(3)
+    sing = r.standard_normal((50, 50))
+    sing[20] = sing[3]
+    indef = np.eye(30)
+    indef[10, 10] = -1.0
+    good = r.standard_normal((40, 40))
+    systems = [(batch.KIND_LU, sing, r.standard_normal((1, 50))),
+               (batch.KIND_CHOL, indef, r.standard_normal((1, 30))),
+               (batch.KIND_LU, good, r.standard_normal((1, 40)))]
+    out = batch.solve_batch(systems)
+    assert out["status"] == [4, 2, 0]  # DivByZero, DecompFailure, Ok
+
+
+def test_batch_mixed():
+    systems = make_systems(24, 21)
+    out = batch.solve_batch(systems, mixed=True)
+    for (kind, a, b), x, st in zip(systems, out["x"], out["status"]):
+        assert st == 0
+        for i in range(b.shape[0]):
+            assert eta(a, x[i], b[i]) <= 1e-13
+
+
+def test_batch_mixed_fallback():
+    """An ill-conditioned system (kappa 1e12) cannot be refined from FP32: the
+    kernel must refactor it in FP64."""
+    a = wl.conditioned(64, 1e12, 1, 2)
+    b = wl.gaussian(1, 64, 3)
+    out = batch.solve_batch([(batch.KIND_LU, a, b)], mixed=True)
+    assert out["status"][0] == 0 and out["fallbacks"] == 1
+    assert eta(a, out["x"][0][0], b[0]) <= 1e-13
+
+
+def test_batch_deterministic():
+    systems = make_systems(12, 31)
+    o1 = batch.solve_batch(systems)
+    o2 = batch.solve_batch(systems[::-1])
+    for x1, x2 in zip(o1["x"], o2["x"][::-1]):
+        assert np.array_equal(x1, x2)
+
+
+# ---------------------------------------------------------------- serving
+
+def test_server_results_alone_and_in_mix():
+    r = wl.rng(41)
+    reqs = []
+    for i in range(16):
+        n = int(np.exp(r.uniform(np.log(16), np.log(700))))
+        kind = batch.KIND_LU if i % 3 else batch.KIND_CHOL
+        if kind == batch.KIND_LU:
+            a = r.standard_normal((n, n))
+        else:
+            g = r.standard_normal((n, n))
+            a = g @ g.T / n + np.eye(n)
+        reqs.append((kind, a, r.standard_normal((2, n))))
+    srv = batch.Server(slots=4)
+    tickets = [srv.submit(k, a, b) for k, a, b in reqs]
+    mix = [srv.wait(t) for t in tickets]
+    srv.close()
+    for (kind, a, b), res in zip(reqs, mix):
+        assert res["status"] == 0
+        check_against_oracle(kind, a, b, res["x"])
+        assert res["latency_ms"] > 0
+    # alone: one slot, one request at a time
+    solo = batch.Server(slots=1)
+    for (kind, a, b), res in zip(reqs, mix):
+        alone = solo.wait(solo.submit(kind, a, b))
+        assert np.array_equal(alone["x"], res["x"])  # bit-identical alone and in the mix
+    solo.close()
+
+
+def test_server_resident_factor_reuse_and_eviction():
+    r = wl.rng(43)
+    n = 300
+    a = r.standard_normal((n, n))
+    b1, b2 = r.standard_normal((1, n)), r.standard_normal((3, n))
+    srv = batch.Server(slots=2)
+    res1 = srv.wait(srv.submit(batch.KIND_LU, a, b1, factor_id=7))
+    res2 = srv.wait(srv.submit(batch.KIND_LU, None, b2, factor_id=7))
+    assert res1["status"] == 0 and res2["status"] == 0
+    for i in range(3):
+        assert eta(a, res2["x"][i], b2[i]) <= 1e-13
+    assert srv.evict(7) == 0
+    with pytest.raises(RuntimeError):
+        res3 = srv.wait(srv.submit(batch.KIND_LU, None, b1, factor_id=7))
+        if res3["status"] != 0:
+            raise RuntimeError("factor evicted")
+    srv.close()
+
+
+def test_server_admission_budget():
+    """A request that can never fit the memory budget is failed with
+    RLG_E_OOM instead of blocking the queue; smaller requests still run."""
+    r = wl.rng(47)
+    srv = batch.Server(mem_budget_bytes=8 << 20, slots=2)
+    big = r.standard_normal((1024, 1024))
+    small = r.standard_normal((64, 64))
+    t_big = srv.submit(batch.KIND_LU, big, r.standard_normal((1, 1024)))
+    t_small = srv.submit(batch.KIND_LU, small, r.standard_normal((1, 64)))
+    assert srv.wait(t_big)["status"] == 9
+    assert srv.wait(t_small)["status"] == 0
+    srv.close()
+
+
+def test_server_mixed_mode():
+    r = wl.rng(53)
+    srv = batch.Server(slots=2, mixed=True)
+    n = 400
+    a = wl.conditioned(n, 1e3, 5, 6)
+    b = r.standard_normal((1, n))
+    res = srv.wait(srv.submit(batch.KIND_LU, a, b))
+    srv.close()
+    assert res["status"] == 0 and res["backward_error"] <= 1e-13
+    assert eta(a, res["x"][0], b[0]) <= 1e-13
