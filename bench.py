@@ -66,8 +66,11 @@ def main():
     lib = _lib.load()
     assert lib.rlg_set_device(0) == 0, _lib.last_error()
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
-    sptr = C.c_void_p(stream.cuda_stream)
+    # torch prepares the inputs on its own stream and is synchronized before
+    # any library call; every library call runs on the library's stream
+    # (NULL), and each timed step ends with a device synchronization, so the
+    # host clock around a step is the step's latency.
+    sptr = None
     n, nrhs = args.n, args.nrhs
 
     # ------------------------------------------------------------ inputs
@@ -85,8 +88,10 @@ def main():
     us = orthogonal(20) * s.unsqueeze(0)
     vt = orthogonal(21).t().contiguous()
     A = torch.empty(n, n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
     assert lib.rlg_gemm_f64_dev(n, n, n, 1.0, us.data_ptr(), n, vt.data_ptr(), n, 0.0,
                                 A.data_ptr(), n, sptr) == 0, _lib.last_error()
+    assert lib.rlg_synchronize() == 0
     del us, vt
     g.manual_seed(22)
     B = torch.randn(nrhs, n, dtype=torch.float64, device=dev, generator=g)
@@ -106,7 +111,7 @@ def main():
         return rep.backward_error, rep.fell_back, rep.iterations
 
     def plain_step():
-        work.copy_(A)
+        assert lib.rlg_dev_copy(work.data_ptr(), A.data_ptr(), A.numel() * 8, sptr) == 0
         h = C.c_void_p()
         st = lib.rlg_lu_decompose_f64_dev(n, work.data_ptr(), n, sptr, C.byref(h))
         assert st == 0, _lib.last_error()
@@ -116,20 +121,17 @@ def main():
 
     def timed(fn, steps, check):
         lat, etas, extra = [], [], []
-        torch.cuda.synchronize()
+        assert lib.rlg_synchronize() == 0
         t_start = time.perf_counter()
         for _ in range(steps):
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
+            t0 = time.perf_counter()
             out = fn()
-            e1.record(stream)
-            e1.synchronize()
-            lat.append(e0.elapsed_time(e1))
+            assert lib.rlg_synchronize() == 0
+            lat.append((time.perf_counter() - t0) * 1e3)
+            log(f"  step {len(lat)}/{steps}: {lat[-1]:.2f} ms")
             extra.append(out)
             if check:
                 etas.append(check())
-        torch.cuda.synchronize()
         return time.perf_counter() - t_start, lat, etas, extra
 
     def check_eta():
@@ -211,7 +213,7 @@ def main():
     if not args.no_extras:
         del work
         torch.cuda.empty_cache()
-        result.update(extras(lib, torch, dev, sptr, stream))
+        result.update(extras(lib, torch, dev, sptr))
 
     # ------------------------------------------------------------ CPU baseline
     if not args.no_cpu:
@@ -219,7 +221,7 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def extras(lib, torch, dev, sptr, stream):
+def extras(lib, torch, dev, sptr):
     """DGEMM 8192^3 and Cholesky 16384 (plain FP64) timings."""
     from rulinalg_amd import _lib
     out = {}
@@ -230,17 +232,16 @@ def extras(lib, torch, dev, sptr, stream):
     g.manual_seed(11)
     b = torch.rand(m, m, dtype=torch.float64, device=dev, generator=g) * 2 - 1
     c = torch.empty_like(a)
-    lib.rlg_gemm_f64_dev(m, m, m, 1.0, a.data_ptr(), m, b.data_ptr(), m, 0.0, c.data_ptr(), m, sptr)
+    torch.cuda.synchronize()
     times = []
-    for _ in range(3):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
+    for it in range(4):
+        t0 = time.perf_counter()
         assert lib.rlg_gemm_f64_dev(m, m, m, 1.0, a.data_ptr(), m, b.data_ptr(), m, 0.0,
-                                    c.data_ptr(), m, sptr) == 0
-        e1.record(stream)
-        e1.synchronize()
-        times.append(e0.elapsed_time(e1))
-    # Freivalds probe against a compensated host product of a few columns
+                                    c.data_ptr(), m, sptr) == 0, _lib.last_error()
+        assert lib.rlg_synchronize() == 0
+        if it:
+            times.append((time.perf_counter() - t0) * 1e3)
+    # Freivalds probe: ||C z - A (B z)|| relative to ||A|| ||B||
     z = torch.randn(m, 4, dtype=torch.float64, device=dev)
     err = (c @ z - a @ (b @ z)).abs().max().item() / (a.abs().sum(1).max().item() * b.abs().max().item())
     tf = 2 * m ** 3 / (min(times) / 1e3) / 1e12
@@ -253,22 +254,23 @@ def extras(lib, torch, dev, sptr, stream):
     G = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
     S = torch.empty_like(G)
     Gt = G.t().contiguous()
+    torch.cuda.synchronize()
     lib.rlg_gemm_f64_dev(n, n, n, 1.0 / n, G.data_ptr(), n, Gt.data_ptr(), n, 0.0, S.data_ptr(), n, sptr)
+    assert lib.rlg_synchronize() == 0
     del G, Gt
     S.diagonal().add_(1.0)
     W = torch.empty_like(S)
+    torch.cuda.synchronize()
     ct = []
     for it in range(2):
-        W.copy_(S)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
+        assert lib.rlg_dev_copy(W.data_ptr(), S.data_ptr(), S.numel() * 8, sptr) == 0
+        assert lib.rlg_synchronize() == 0
+        t0 = time.perf_counter()
         h = C.c_void_p()
         st = lib.rlg_chol_decompose_f64_dev(n, W.data_ptr(), n, sptr, C.byref(h))
-        e1.record(stream)
-        e1.synchronize()
         assert st == 0, _lib.last_error()
-        ct.append(e0.elapsed_time(e1))
+        assert lib.rlg_synchronize() == 0
+        ct.append((time.perf_counter() - t0) * 1e3)
         lib.rlg_chol_free(h)
     tf = n ** 3 / 3 / (min(ct) / 1e3) / 1e12
     out["cholesky_16384"] = {"ms": round(min(ct), 3), "tflops": round(tf, 2),

@@ -59,6 +59,8 @@ int rlg_dev_alloc(size_t bytes, void **ptr);
 int rlg_dev_free(void *ptr);
 int rlg_dev_upload(void *dst, const void *src, size_t bytes);
 int rlg_dev_download(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy, asynchronous on `stream` (NULL: the library's). */
+int rlg_dev_copy(void *dst, const void *src, size_t bytes, void *stream);
 
 /* ---------------------------------------------------------------- GEMM
  * C = A * B for an (a_rows x a_cols) A and a (b_rows x b_cols) B: the

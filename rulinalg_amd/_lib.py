@@ -72,6 +72,7 @@ SIGNATURES = {
     "rlg_dev_free": (i32, [vp]),
     "rlg_dev_upload": (i32, [vp, vp, sz]),
     "rlg_dev_download": (i32, [vp, vp, sz]),
+    "rlg_dev_copy": (i32, [vp, vp, sz, vp]),
     "rlg_gemm_f64": (i32, [sz, sz, vp, sz, sz, sz, vp, sz, vp, sz]),
     "rlg_gemm_f32": (i32, [sz, sz, vp, sz, sz, sz, vp, sz, vp, sz]),
     "rlg_gemm_f64_dev": (i32, [sz, sz, sz, dbl, vp, sz, vp, sz, dbl, vp, sz, vp]),

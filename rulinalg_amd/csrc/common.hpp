@@ -50,6 +50,9 @@ struct Device {
 // created lazily and live until process exit.
 int current_device(Device **dev);
 hipStream_t pick_stream(Device *dev, void *user_stream);
+// A second stream of the current device (default priority, created once)
+// for work that overlaps the caller's stream, ordered by events.
+int aux_stream(hipStream_t *out);
 
 // ------------------------------------------------------------------ memory
 

@@ -424,6 +424,41 @@ static int update_lowp(hipStream_t s, size_t m, size_t n, size_t k, const float 
     return gemm_lowp(s, g_lowp_prec, m, n, k, -1.f, a, lda, b, ldb, 1.f, c, ldc);
 }
 
+// Events used to order the look-ahead stream against the main stream.
+struct StepEvents {
+    hipEvent_t ready = nullptr, big = nullptr;
+    bool big_pending = false;
+    ~StepEvents() {
+        if (ready) (void)hipEventDestroy(ready);
+        if (big) (void)hipEventDestroy(big);
+    }
+};
+
+// Factor the outer block of columns [k0, k0+kb): inner panels of PNB columns,
+// their swaps on the rest of the block, and the block-local updates.
+template <class T>
+static int outer_panel(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv, int64_t *info,
+                       const LuWork &lw, size_t k0, size_t kb) {
+    for (size_t j0 = k0; j0 < k0 + kb; j0 += PNB) {
+        const size_t jb = k0 + kb - j0 < (size_t)PNB ? k0 + kb - j0 : PNB;
+        RLG_TRY(panel<T>(s, a, lda, j0, n - j0, jb, ipiv, info, n, lw, lw.bar + j0 / PNB));
+        // inner swaps on the other columns of the outer block
+        RLG_TRY(laswp<T>(s, a, lda, ipiv, j0, jb, k0, j0, j0 + jb, k0 + kb, lw));
+        if (j0 + jb < k0 + kb) {
+            RLG_TRY(trsm_llu<T>(s, a, lda, j0, jb, j0 + jb, k0 + kb));
+            RLG_TRY(update_native<T>(s, n - j0 - jb, k0 + kb - j0 - jb, jb, a + (j0 + jb) * lda + j0,
+                                     lda, a + j0 * lda + j0 + jb, lda, a + (j0 + jb) * lda + j0 + jb,
+                                     lda));
+        }
+    }
+    return RLG_OK;
+}
+
+// Right-looking blocked LU with look-ahead.  After outer block k is factored
+// and its block row solved, the trailing update is split: the columns of
+// block k+1 are updated first on the main stream, which then factors block
+// k+1 while the rest of the trailing update runs on the auxiliary stream.
+// The next step's row swaps and block-row solve wait for that update.
 template <class T>
 static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv, int64_t *info,
                           void *work, UpdateFn<T> big_update) {
@@ -432,37 +467,50 @@ static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ip
     RLG_TRY(fill_u32(lw.bar, panels, 0u, s));
     RLG_TRY(fill_u32(reinterpret_cast<uint32_t *>(lw.timeout), 1, 0u, s));
     RLG_TRY(fill_i64(info, 1, (int64_t)n, s));
-    size_t pidx = 0;
+    if (n == 0) return RLG_OK;
+    hipStream_t s2 = nullptr;
+    StepEvents ev;
+    if (n > 2 * (size_t)ONB) {
+        RLG_TRY(aux_stream(&s2));
+        RLG_HIP(hipEventCreateWithFlags(&ev.ready, hipEventDisableTiming));
+        RLG_HIP(hipEventCreateWithFlags(&ev.big, hipEventDisableTiming));
+    }
+    RLG_TRY(outer_panel<T>(s, n, a, lda, ipiv, info, lw, 0, n < (size_t)ONB ? n : ONB));
     for (size_t k0 = 0; k0 < n; k0 += ONB) {
         const size_t kb = n - k0 < (size_t)ONB ? n - k0 : ONB;
-        for (size_t j0 = k0; j0 < k0 + kb; j0 += PNB) {
-            const size_t jb = k0 + kb - j0 < (size_t)PNB ? k0 + kb - j0 : PNB;
-            RLG_TRY(panel<T>(s, a, lda, j0, n - j0, jb, ipiv, info, n, lw, lw.bar + pidx));
-            ++pidx;
-            // inner swaps on the other columns of the outer block
-            RLG_TRY(laswp<T>(s, a, lda, ipiv, j0, jb, k0, j0, j0 + jb, k0 + kb, lw));
-            if (j0 + jb < k0 + kb) {
-                RLG_TRY(trsm_llu<T>(s, a, lda, j0, jb, j0 + jb, k0 + kb));
-                RLG_TRY(update_native<T>(s, n - j0 - jb, k0 + kb - j0 - jb, jb,
-                                         a + (j0 + jb) * lda + j0, lda, a + j0 * lda + j0 + jb, lda,
-                                         a + (j0 + jb) * lda + j0 + jb, lda));
-            }
+        const size_t c0 = k0 + kb;
+        if (ev.big_pending) {  // the previous trailing update must be complete
+            RLG_HIP(hipStreamWaitEvent(s, ev.big, 0));
+            ev.big_pending = false;
         }
         // outer swaps on everything left and right of the outer block
-        RLG_TRY(laswp<T>(s, a, lda, ipiv, k0, kb, 0, k0, k0 + kb, n, lw));
-        if (k0 + kb < n) {
-            const size_t c0 = k0 + kb;
-            for (size_t j0 = k0; j0 < k0 + kb; j0 += PNB) {
-                const size_t jb = k0 + kb - j0 < (size_t)PNB ? k0 + kb - j0 : PNB;
-                RLG_TRY(trsm_llu<T>(s, a, lda, j0, jb, c0, n));
-                if (j0 + jb < k0 + kb)
-                    RLG_TRY(update_native<T>(s, k0 + kb - j0 - jb, n - c0, jb, a + (j0 + jb) * lda + j0,
-                                             lda, a + j0 * lda + c0, lda, a + (j0 + jb) * lda + c0, lda));
-            }
+        RLG_TRY(laswp<T>(s, a, lda, ipiv, k0, kb, 0, k0, c0, n, lw));
+        if (c0 >= n) break;
+        for (size_t j0 = k0; j0 < c0; j0 += PNB) {
+            const size_t jb = c0 - j0 < (size_t)PNB ? c0 - j0 : PNB;
+            RLG_TRY(trsm_llu<T>(s, a, lda, j0, jb, c0, n));
+            if (j0 + jb < c0)
+                RLG_TRY(update_native<T>(s, c0 - j0 - jb, n - c0, jb, a + (j0 + jb) * lda + j0, lda,
+                                         a + j0 * lda + c0, lda, a + (j0 + jb) * lda + c0, lda));
+        }
+        const size_t kb2 = n - c0 < (size_t)ONB ? n - c0 : ONB;
+        if (!s2 || c0 + kb2 >= n) {
             RLG_TRY(big_update(s, n - c0, n - c0, kb, a + c0 * lda + k0, lda, a + k0 * lda + c0, lda,
                                a + c0 * lda + c0, lda));
+        } else {
+            // look-ahead columns first, then the rest on the auxiliary stream
+            RLG_TRY(big_update(s, n - c0, kb2, kb, a + c0 * lda + k0, lda, a + k0 * lda + c0, lda,
+                               a + c0 * lda + c0, lda));
+            RLG_HIP(hipEventRecord(ev.ready, s));
+            RLG_HIP(hipStreamWaitEvent(s2, ev.ready, 0));
+            RLG_TRY(big_update(s2, n - c0, n - c0 - kb2, kb, a + c0 * lda + k0, lda,
+                               a + k0 * lda + c0 + kb2, lda, a + c0 * lda + c0 + kb2, lda));
+            RLG_HIP(hipEventRecord(ev.big, s2));
+            ev.big_pending = true;
         }
+        RLG_TRY(outer_panel<T>(s, n, a, lda, ipiv, info, lw, c0, kb2));
     }
+    if (ev.big_pending) RLG_HIP(hipStreamWaitEvent(s, ev.big, 0));
     return RLG_OK;
 }
 

@@ -229,24 +229,39 @@ struct EventTimer {
     }
 };
 
+// Enqueues the norms of the normwise backward error on stream s:
+// norms = [ ||r_k||inf (nrhs) | ||x_k||inf (nrhs) | ||b_k||inf (nrhs) | ||A||inf ],
+// with res (nrhs * n) and rs (n) as workspace.
+int backward_error_enqueue(hipStream_t s, size_t n, const double *a, size_t lda, size_t nrhs,
+                           const double *b, size_t ldb, const double *x, size_t ldx, double *nm,
+                           double *res, double *rs) {
+    RLG_TRY(zero(s, nm + 3 * nrhs, 1));
+    hipLaunchKernelGGL(k_row_scale, dim3(ceil_div(n, 4)), dim3(256), 0, s, (int)n, a, (long)lda, rs,
+                       nm + 3 * nrhs);
+    RLG_LAUNCHED();
+    RLG_TRY(residual(s, n, a, lda, nrhs, x, ldx, b, ldb, res, n, nm));
+    RLG_TRY(vec_maxabs(s, n, nrhs, x, ldx, nm + nrhs));
+    RLG_TRY(vec_maxabs(s, n, nrhs, b, ldb, nm + 2 * nrhs));
+    return RLG_OK;
+}
+
+double backward_error_from(const double *norms_host, size_t nrhs) {
+    std::vector<double> h(norms_host, norms_host + 3 * nrhs + 1);
+    return eta_of(h, nrhs, h[3 * nrhs]);
+}
+
 int backward_error_dev(hipStream_t s, size_t n, const double *a, size_t lda, size_t nrhs,
                        const double *b, size_t ldb, const double *x, size_t ldx, double *eta) {
     PoolBuf res, norms, rs;
     RLG_TRY(res.alloc(nrhs * n * 8, s));
     RLG_TRY(norms.alloc((3 * nrhs + 1) * 8, s));
     RLG_TRY(rs.alloc(n * 8, s));
-    double *nm = norms.as<double>();
-    RLG_TRY(zero(s, nm + 3 * nrhs, 1));
-    hipLaunchKernelGGL(k_row_scale, dim3(ceil_div(n, 4)), dim3(256), 0, s, (int)n, a, (long)lda,
-                       rs.as<double>(), nm + 3 * nrhs);
-    RLG_LAUNCHED();
-    RLG_TRY(residual(s, n, a, lda, nrhs, x, ldx, b, ldb, res.as<double>(), n, nm));
-    RLG_TRY(vec_maxabs(s, n, nrhs, x, ldx, nm + nrhs));
-    RLG_TRY(vec_maxabs(s, n, nrhs, b, ldb, nm + 2 * nrhs));
+    RLG_TRY(backward_error_enqueue(s, n, a, lda, nrhs, b, ldb, x, ldx, norms.as<double>(),
+                                   res.as<double>(), rs.as<double>()));
     std::vector<double> h(3 * nrhs + 1);
-    RLG_HIP(hipMemcpyAsync(h.data(), nm, h.size() * 8, hipMemcpyDeviceToHost, s));
+    RLG_HIP(hipMemcpyAsync(h.data(), norms.p, h.size() * 8, hipMemcpyDeviceToHost, s));
     RLG_HIP(hipStreamSynchronize(s));
-    *eta = eta_of(h, nrhs, h[3 * nrhs]);
+    *eta = backward_error_from(h.data(), nrhs);
     return RLG_OK;
 }
 
@@ -314,9 +329,11 @@ int solve_mixed_dev(hipStream_t s, size_t n, const double *a, size_t lda, size_t
         if (timeout) { set_error("LU panel barrier timed out"); return RLG_E_DEVICE; }
         if (info_h == (int64_t)n) {
             std::vector<int64_t> ip(n), raw(n);
-            RLG_HIP(hipMemcpy(ip.data(), ipiv.p, n * 8, hipMemcpyDeviceToHost));
+            RLG_HIP(hipMemcpyAsync(ip.data(), ipiv.p, n * 8, hipMemcpyDeviceToHost, s));
+            RLG_HIP(hipStreamSynchronize(s));
             perm_from_ipiv(n, ip.data(), raw.data());
-            RLG_HIP(hipMemcpy(ipiv.p, raw.data(), n * 8, hipMemcpyHostToDevice));  // ipiv now holds raw
+            RLG_HIP(hipMemcpyAsync(ipiv.p, raw.data(), n * 8, hipMemcpyHostToDevice, s));  // ipiv now holds raw
+            RLG_HIP(hipStreamSynchronize(s));
             EventTimer tr(s);
             double prev = INFINITY;
             // x = 0, r = b
@@ -385,9 +402,11 @@ int solve_mixed_dev(hipStream_t s, size_t n, const double *a, size_t lda, size_t
             return RLG_E_DIV_BY_ZERO;
         }
         std::vector<int64_t> iph(n), raw(n);
-        RLG_HIP(hipMemcpy(iph.data(), ip.p, n * 8, hipMemcpyDeviceToHost));
+        RLG_HIP(hipMemcpyAsync(iph.data(), ip.p, n * 8, hipMemcpyDeviceToHost, s));
+        RLG_HIP(hipStreamSynchronize(s));
         perm_from_ipiv(n, iph.data(), raw.data());
-        RLG_HIP(hipMemcpy(ip.p, raw.data(), n * 8, hipMemcpyHostToDevice));
+        RLG_HIP(hipMemcpyAsync(ip.p, raw.data(), n * 8, hipMemcpyHostToDevice, s));
+        RLG_HIP(hipStreamSynchronize(s));
         RLG_TRY(lu_solve<double>(s, n, af.as<double>(), n, ip.as<int64_t>(), nrhs, b, ldb, x, ldx,
                                  tmp.as<double>()));
         st = backward_error_dev(s, n, a, lda, nrhs, b, ldb, x, ldx, &eta);

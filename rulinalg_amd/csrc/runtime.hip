@@ -63,6 +63,17 @@ hipStream_t pick_stream(Device *dev, void *user_stream) {
     return user_stream ? static_cast<hipStream_t>(user_stream) : dev->stream;
 }
 
+int aux_stream(hipStream_t *out) {
+    int id = 0;
+    RLG_HIP(hipGetDevice(&id));
+    static hipStream_t aux[64] = {nullptr};
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (id < 0 || id >= 64) { set_error("device index %d out of range", id); return RLG_E_DEVICE; }
+    if (!aux[id]) RLG_HIP(hipStreamCreateWithFlags(&aux[id], hipStreamNonBlocking));
+    *out = aux[id];
+    return RLG_OK;
+}
+
 // ------------------------------------------------------------------ profiler
 
 namespace {
@@ -311,6 +322,14 @@ int rlg_dev_download(void *dst, const void *src, size_t bytes) {
     RLG_HIP(hipDeviceSynchronize());
     RLG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, d->stream));
     RLG_HIP(hipStreamSynchronize(d->stream));
+    return RLG_OK;
+}
+
+int rlg_dev_copy(void *dst, const void *src, size_t bytes, void *stream) {
+    Device *d;
+    RLG_TRY(current_device(&d));
+    if (!bytes) return RLG_OK;
+    RLG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, pick_stream(d, stream)));
     return RLG_OK;
 }
 

@@ -72,17 +72,21 @@ def check_against_oracle(kind, a, b, x, factor=None, ipiv=None):
 
 
 def test_batch_small_bit_exact():
-    """Orders <= 16 fit one panel (NB = 16): factors and pivots bit-identical."""
+    """Orders <= 8 fit one panel (every tuned panel width is >= 8): factors and
+    pivots bit-identical; wider ones within the factor tolerance."""
     r = wl.rng(5)
     systems = []
-    for n in [1, 2, 3, 8, 13, 16]:
+    for n in [1, 2, 3, 5, 8, 13, 16, 40]:
         systems.append((batch.KIND_LU, r.standard_normal((n, n)), r.standard_normal((2, n))))
     out = batch.solve_batch(systems)
     for (kind, a, b), f, ip, st in zip(systems, out["factor"], out["ipiv"], out["status"]):
         assert st == 0
         lu_or, perm_or, ipiv_or = orc.lu_decompose(a)
         assert np.array_equal(ip, ipiv_or)
-        assert np.array_equal(f, lu_or)
+        if a.shape[0] <= 8:
+            assert np.array_equal(f, lu_or)
+        else:
+            assert np.abs(f - lu_or).max() / np.abs(lu_or).max() <= 1e-10
 
 
 def test_batch_plain_vs_oracle():
@@ -95,9 +99,13 @@ def test_batch_plain_vs_oracle():
 
 
 def test_batch_failures_reported_per_item():
+    # A zero column stays exactly zero under any update order, so step 30 meets
+    # an exactly zero pivot as in the reference (lu.rs:179-183).  (A zero pivot
+    # that only arises through cancellation can come out as a rounding residue
+    # above epsilon once the update order differs from the reference's.)
     r = wl.rng(3)
     sing = r.standard_normal((50, 50))
-    sing[20] = sing[3]
+    sing[:, 30] = 0.0
     indef = np.eye(30)
     indef[10, 10] = -1.0
     good = r.standard_normal((40, 40))
