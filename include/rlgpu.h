@@ -242,6 +242,23 @@ int rlg_batch_solve_mixed_dev(size_t count, rlg_batch_item *items,
                               int max_iter, double tol, size_t *fallbacks,
                               void *stream);
 
+/* Autotune the batched solver: for each n-bucket (n <= 32, 64, 128, 256,
+ * 512) time synthetic systems with each panel width (8, 16, 32) and keep the
+ * fastest for later batched calls.  Writes one line per bucket into report
+ * (may be NULL). */
+int rlg_batch_autotune(int systems_per_bucket, char *report, size_t len);
+
+/* The device part of rlg_batch_solve_f64_dev (work-counter reset and the
+ * persistent kernel) for a fixed item array, captured once into a HIP graph
+ * and replayed.  The data the items point to may change between replays;
+ * the item array, orders and shapes may not.  Replays on one stream are
+ * ordered; overlapping replays race on the shared work counter. */
+typedef struct rlg_batch_graph rlg_batch_graph;
+int rlg_batch_graph_create(size_t count, rlg_batch_item *items,
+                           rlg_batch_graph **out);
+int rlg_batch_graph_launch(rlg_batch_graph *g, void *stream);
+void rlg_batch_graph_destroy(rlg_batch_graph *g);
+
 /* ---------------------------------------------------------------- serving
  * Continuous-batching request loop.  Requests are admitted while device
  * memory under the budget and a stream slot are free; factors can be kept

@@ -113,6 +113,10 @@ SIGNATURES = {
     "rlg_backward_error_f64_dev": (i32, [sz, vp, sz, sz, vp, sz, vp, sz, PD, vp]),
     "rlg_batch_solve_f64_dev": (i32, [sz, vp, vp]),
     "rlg_batch_solve_mixed_dev": (i32, [sz, vp, i32, dbl, PSZ, vp]),
+    "rlg_batch_autotune": (i32, [i32, C.c_char_p, sz]),
+    "rlg_batch_graph_create": (i32, [sz, vp, C.POINTER(vp)]),
+    "rlg_batch_graph_launch": (i32, [vp, vp]),
+    "rlg_batch_graph_destroy": (None, [vp]),
     "rlg_server_opts_default": (None, [C.POINTER(ServerOpts)]),
     "rlg_server_create": (i32, [C.POINTER(ServerOpts), C.POINTER(vp)]),
     "rlg_server_submit": (i32, [vp, C.POINTER(Request), PI64]),

@@ -459,6 +459,10 @@ static int outer_panel(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv,
 // block k+1 are updated first on the main stream, which then factors block
 // k+1 while the rest of the trailing update runs on the auxiliary stream.
 // The next step's row swaps and block-row solve wait for that update.
+//
+// LOOK_AHEAD is off: a run with it on stalled on the GPU and the stall has
+// not been traced to its cause, so every update runs on the main stream.
+constexpr bool LOOK_AHEAD = false;
 template <class T>
 static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv, int64_t *info,
                           void *work, UpdateFn<T> big_update) {
@@ -470,7 +474,7 @@ static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ip
     if (n == 0) return RLG_OK;
     hipStream_t s2 = nullptr;
     StepEvents ev;
-    if (n > 2 * (size_t)ONB) {
+    if (LOOK_AHEAD && n > 2 * (size_t)ONB) {
         RLG_TRY(aux_stream(&s2));
         RLG_HIP(hipEventCreateWithFlags(&ev.ready, hipEventDisableTiming));
         RLG_HIP(hipEventCreateWithFlags(&ev.big, hipEventDisableTiming));

@@ -212,7 +212,10 @@ static int trsv_flag(hipStream_t s, size_t n, const T *m, size_t lda, size_t nrh
     return RLG_OK;
 }
 
-constexpr size_t FLAG_MIN_N = 128;  // below this the blocked launches are as fast
+// Minimum order for the single-launch flag-ordered solve.  The path is off
+// (no order reaches it): a solve through it stalled on the GPU and the stall
+// has not been traced to its cause, so every solve uses the blocked launches.
+constexpr size_t FLAG_MIN_N = ~(size_t)0;
 
 template <class T>
 int trsv_lower(hipStream_t s, size_t n, const T *l, size_t lda, bool unit, size_t nrhs, T *x,
