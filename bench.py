@@ -10,8 +10,9 @@ step's backward error is checked against 1e-13.
 
 Prints one JSON line: value = mixed solves/s over the timed steps, with
 p50/p99 step latency, the plain path's numbers beside it, the roofline of
-the dominant kernel (from HIP events around every launch in the timed
-region), and the single-core CPU baseline (the oracle's restatement of
+the dominant kernel (from HIP events around every launch, recorded on the
+library's stream during a second pass over the same steps, so the timed
+steps carry no event overhead), and the single-core CPU baseline (the oracle's restatement of
 rulinalg's PartialPivLu::decompose + solve, timed on a bounded sample and
 scaled by n^3).
 """
@@ -47,7 +48,29 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extras", action="store_true")
     p.add_argument("--cpu-n", type=int, default=2048)
+    p.add_argument("--prof-steps", type=int, default=None,
+                   help="steps of the profiled pass (default: --steps)")
+    p.add_argument("--gen", default="qr", choices=["qr", "householder"],
+                   help="orthogonal factors from QR of Gaussian matrices (the workload) or "
+                        "from 8 Householder reflectors each (for counter runs)")
+    p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                   help="only run three device copies of MIB MiB (the plain copy kernel that "
+                        "calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
     return p.parse_args()
+
+
+def copy_probe(mib):
+    """Three launches of torch's device copy over `mib` MiB: each reads and
+    writes exactly mib MiB, the known size the HBM counters are scaled by."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8, "launches": 3}), flush=True)
 
 
 def log(msg):
@@ -60,6 +83,9 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU layout is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     import torch
 
     from rulinalg_amd import _lib
@@ -85,14 +111,30 @@ def main():
         return q
 
     s = torch.logspace(0.0, float(np.log10(args.kappa)), n, dtype=torch.float64, device=dev)
-    us = orthogonal(20) * s.unsqueeze(0)
-    vt = orthogonal(21).t().contiguous()
     A = torch.empty(n, n, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-    assert lib.rlg_gemm_f64_dev(n, n, n, 1.0, us.data_ptr(), n, vt.data_ptr(), n, 0.0,
-                                A.data_ptr(), n, sptr) == 0, _lib.last_error()
-    assert lib.rlg_synchronize() == 0
-    del us, vt
+    if args.gen == "qr":
+        us = orthogonal(20) * s.unsqueeze(0)
+        vt = orthogonal(21).t().contiguous()
+        torch.cuda.synchronize()
+        assert lib.rlg_gemm_f64_dev(n, n, n, 1.0, us.data_ptr(), n, vt.data_ptr(), n, 0.0,
+                                    A.data_ptr(), n, sptr) == 0, _lib.last_error()
+        assert lib.rlg_synchronize() == 0
+        del us, vt
+    else:
+        # U and V as products of 8 Householder reflectors each (exactly
+        # orthogonal, same singular values; no LAPACK call involved)
+        A.zero_()
+        A.diagonal().copy_(s)
+        g.manual_seed(20)
+        for side in (0, 1):
+            for _ in range(8):
+                v = torch.randn(n, dtype=torch.float64, device=dev, generator=g)
+                v /= v.norm()
+                if side == 0:   # A <- (I - 2 v v^T) A
+                    A.addr_(v, A.t().mv(v), alpha=-2.0)
+                else:           # A <- A (I - 2 v v^T)
+                    A.addr_(A.mv(v), v, alpha=-2.0)
+        torch.cuda.synchronize()
     g.manual_seed(22)
     B = torch.randn(nrhs, n, dtype=torch.float64, device=dev, generator=g)
     X = torch.empty_like(B)
@@ -144,10 +186,16 @@ def main():
     log(f"warmup {args.warmup} mixed step(s), prec={args.prec}")
     for _ in range(args.warmup):
         mixed_step()
-    lib.rlg_prof_reset()
-    lib.rlg_prof_enable(1)
     log(f"timing {args.steps} mixed steps")
     wall, lat, etas, extra = timed(mixed_step, args.steps, check_eta)
+    # the same steps again with a HIP event pair around every launch on the
+    # library's stream: per-kernel-class durations, launches and algorithmic
+    # work (the step time above is measured without these events)
+    prof_steps = args.prof_steps if args.prof_steps is not None else args.steps
+    log(f"profiling {prof_steps} mixed steps (per-launch events)")
+    lib.rlg_prof_reset()
+    lib.rlg_prof_enable(1)
+    timed(mixed_step, prof_steps, None)
     lib.rlg_prof_enable(0)
     prof = {}
     buf = C.create_string_buffer(1 << 16)
@@ -177,18 +225,24 @@ def main():
     gemm_classes = {"gemm_f64": "f64", "gemm_f32": "f32", "gemm_f16": "f16", "gemm_bf16": "bf16"}
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     name, (cnt, ms, wk) = dom
-    if name in gemm_classes and wk > 0:
-        achieved = wk / (ms / 1e3) / 1e12
-        peak = PEAKS[gemm_classes[name]]
-        roof = {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
-                "launches": cnt, "avg_launch_ms": round(ms / cnt, 5),
-                "flops_per_launch": wk / cnt}
-    else:
-        roof = {"kernel": name, "bound": "hbm", "achieved": None, "peak": HBM_PEAK, "unit": "GB/s",
-                "frac": None, "traffic": None, "launches": cnt, "avg_launch_ms": round(ms / cnt, 5)}
-    breakdown = {k: {"launches": v[0], "ms_per_step": round(v[1] / args.steps, 3)} for k, v in
-                 sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    def roof_of(name, cnt, ms, wk):
+        """achieved = algorithmic work per launch / average launch duration
+        (flops for the GEMM classes, bytes for the others)."""
+        if name in gemm_classes:
+            peak, unit, bound, scale = PEAKS[gemm_classes[name]], "TFLOP/s", "mfma", 1e12
+        else:
+            peak, unit, bound, scale = HBM_PEAK, "GB/s", "hbm", 1e9
+        achieved = wk / (ms / 1e3) / scale if wk > 0 and ms > 0 else None
+        return {"kernel": name, "bound": bound,
+                "achieved": None if achieved is None else round(achieved, 3), "peak": peak,
+                "unit": unit, "frac": None if achieved is None else round(achieved / peak, 4),
+                "traffic": None, "launches": cnt, "avg_launch_ms": round(ms / cnt, 5),
+                ("flops_per_launch" if bound == "mfma" else "bytes_per_launch"): wk / cnt}
+
+    roof = roof_of(name, cnt, ms, wk)
+    breakdown = {k: {"launches": v[0] // prof_steps, "ms_per_step": round(v[1] / prof_steps, 3),
+                     "roofline_frac": roof_of(k, *v)["frac"]}
+                 for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
 
     result = {
         "metric": METRIC, "value": round(value, 4), "unit": "solves/s", "n_gpus": 1,

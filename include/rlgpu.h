@@ -88,6 +88,32 @@ int rlg_gemm_f32_dev(size_t m, size_t k, size_t n, float alpha,
 int rlg_matvec_f64(size_t m, size_t n, const double *a, size_t lda,
                    size_t v_len, const double *v, double *y);
 
+/* ---------------------------------------------------------------- triangular
+ * BaseMatrix::solve_u_triangular / solve_l_triangular
+ * (src/matrix/base/mod.rs:1015-1024, 1058-1067) on nrhs right-hand sides
+ * (vector r at b + r*ldb, solution at x + r*ldx; nrhs = 1 is the reference
+ * call).  Only the upper (resp. lower) triangle is read.  b_len != cols or
+ * rows != cols returns RLG_E_SHAPE (the asserts at base/mod.rs:1018 and
+ * src/matrix/mod.rs:322); a diagonal entry with |m_ii| < epsilon returns
+ * RLG_E_DIV_BY_ZERO (src/matrix/mod.rs:331-335, 375-378). */
+int rlg_solve_u_triangular_f64(size_t rows, size_t cols, const double *u, size_t ldu,
+                               size_t b_len, size_t nrhs, const double *b, size_t ldb,
+                               double *x, size_t ldx);
+int rlg_solve_l_triangular_f64(size_t rows, size_t cols, const double *l, size_t ldl,
+                               size_t b_len, size_t nrhs, const double *b, size_t ldb,
+                               double *x, size_t ldx);
+int rlg_solve_u_triangular_f32(size_t rows, size_t cols, const float *u, size_t ldu,
+                               size_t b_len, size_t nrhs, const float *b, size_t ldb,
+                               float *x, size_t ldx);
+int rlg_solve_l_triangular_f32(size_t rows, size_t cols, const float *l, size_t ldl,
+                               size_t b_len, size_t nrhs, const float *b, size_t ldb,
+                               float *x, size_t ldx);
+/* Device form of both (upper != 0: U x = b, else L x = b); x may equal b.
+ * Waits for the stream before returning, to report RLG_E_DIV_BY_ZERO. */
+int rlg_solve_triangular_f64_dev(int upper, size_t n, const double *m, size_t ldm,
+                                 size_t nrhs, const double *b, size_t ldb, double *x,
+                                 size_t ldx, void *stream);
+
 /* ---------------------------------------------------------------- LU
  * PartialPivLu (src/matrix/decomposition/lu.rs:130-300).  The factor lives
  * on the device: packed L (unit diagonal implied) and U, plus the pivots. */

@@ -138,6 +138,32 @@ class _Operand:
             return Vector(y)
         raise TypeError(f"unsupported operand {type(other)}")
 
+    def _tri_solve(self, y, upper):
+        if not isinstance(y, Vector):
+            y = Vector(y)
+        dt = self.dtype
+        m = np.ascontiguousarray(self.to_numpy(), dtype=dt)
+        b = np.ascontiguousarray(y.data(), dtype=dt)
+        x = np.empty_like(b)
+        lib = _lib.load()
+        f32 = dt == np.float32
+        if upper:
+            fn = lib.rlg_solve_u_triangular_f32 if f32 else lib.rlg_solve_u_triangular_f64
+        else:
+            fn = lib.rlg_solve_l_triangular_f32 if f32 else lib.rlg_solve_l_triangular_f64
+        n = max(self._cols, 1)
+        _check(fn(self._rows, self._cols, m.ctypes.data, n, b.shape[0], 1, b.ctypes.data,
+                  max(b.shape[0], 1), x.ctypes.data, max(b.shape[0], 1)))
+        return Vector(x)
+
+    def solve_u_triangular(self, y):
+        """BaseMatrix::solve_u_triangular (src/matrix/base/mod.rs:1015-1024)."""
+        return self._tri_solve(y, True)
+
+    def solve_l_triangular(self, y):
+        """BaseMatrix::solve_l_triangular (src/matrix/base/mod.rs:1058-1067)."""
+        return self._tri_solve(y, False)
+
 
 class Matrix(_Operand):
     """Matrix<T> (src/matrix/mod.rs:45-49): row-major, row_stride == cols."""

@@ -244,10 +244,12 @@ int chol_solve_host(const rlg_chol *h, size_t b_len, size_t nrhs, const T *b, si
     Device *d;
     RLG_TRY(current_device(&d));
     hipStream_t s = d->stream;
-    PoolBuf db;
+    PoolBuf db, tmp;
     RLG_TRY(db.alloc(nrhs * n * sizeof(T), s));
+    RLG_TRY(tmp.alloc(nrhs * n * sizeof(T), s));
     RLG_TRY(upload_matrix<T>(s, db.as<T>(), n, b, ldb, nrhs, n));
-    RLG_TRY(chol_solve<T>(s, n, static_cast<const T *>(h->a), h->lda, nrhs, db.as<T>(), n, db.as<T>(), n));
+    RLG_TRY(chol_solve<T>(s, n, static_cast<const T *>(h->a), h->lda, nrhs, db.as<T>(), n, db.as<T>(), n,
+                          tmp.as<T>()));
     RLG_TRY(download_matrix<T>(s, x, ldx, db.as<T>(), n, nrhs, n));
     RLG_HIP(hipStreamSynchronize(s));
     return RLG_OK;
@@ -319,6 +321,110 @@ int rlg_matvec_f64(size_t m, size_t n, const double *a, size_t lda, size_t v_len
     RLG_HIP(hipMemcpyAsync(y, dy.p, m * 8, hipMemcpyDeviceToHost, s));
     RLG_HIP(hipStreamSynchronize(s));
     return RLG_OK;
+}
+
+// ------------------------------------------------------------------ triangular
+
+}  // extern "C"
+
+namespace {
+
+// solve_u_triangular / solve_l_triangular (src/matrix/base/mod.rs:1015-1024,
+// 1058-1067): the vector must match the column count (panic), the
+// substitution needs a square matrix (panic), and a diagonal entry below
+// epsilon in magnitude fails with DivByZero (src/matrix/mod.rs:331-335,
+// 375-378).  Only the triangle named by `upper` is read.
+template <class T>
+int tri_solve_dev(hipStream_t s, bool upper, size_t n, const T *m, size_t ldm, size_t nrhs,
+                  const T *b, size_t ldb, T *x, size_t ldx) {
+    PoolBuf flag, tmp;
+    RLG_TRY(flag.alloc(4, s));
+    RLG_TRY(tmp.alloc(nrhs * n * sizeof(T), s));
+    RLG_TRY(fill_u32(flag.as<uint32_t>(), 1, 0u, s));
+    RLG_TRY(diag_tiny<T>(s, n, m, ldm, std::numeric_limits<T>::epsilon(), flag.as<uint32_t>()));
+    // b goes to tmp first (x may alias b); tmp is the running residual
+    RLG_HIP(hipMemcpy2DAsync(tmp.p, n * sizeof(T), b, ldb * sizeof(T), n * sizeof(T), nrhs,
+                             hipMemcpyDeviceToDevice, s));
+    if (upper)
+        RLG_TRY(trsv_upper<T>(s, n, m, ldm, nrhs, tmp.as<T>(), n, x, ldx));
+    else
+        RLG_TRY(trsv_lower<T>(s, n, m, ldm, false, nrhs, tmp.as<T>(), n, x, ldx));
+    uint32_t tiny = 0;
+    RLG_HIP(hipMemcpyAsync(&tiny, flag.p, 4, hipMemcpyDeviceToHost, s));
+    RLG_HIP(hipStreamSynchronize(s));
+    if (tiny) {
+        set_error("%s triangular matrix is singular to working precision.", upper ? "Upper" : "Lower");
+        return RLG_E_DIV_BY_ZERO;
+    }
+    return RLG_OK;
+}
+
+template <class T>
+int tri_solve_host(bool upper, size_t rows, size_t cols, const T *m, size_t ldm, size_t b_len,
+                   size_t nrhs, const T *b, size_t ldb, T *x, size_t ldx) {
+    if (b_len != cols) {
+        set_error("Vector size %zu != %zu Matrix column count.", b_len, cols);
+        return RLG_E_SHAPE;
+    }
+    if (rows != cols) {
+        set_error("Matrix %s must be square.", upper ? "U" : "L");
+        return RLG_E_SHAPE;
+    }
+    const size_t n = rows;
+    if (n == 0 || nrhs == 0) return RLG_OK;
+    if (!m || !b || !x || ldm < n || ldb < n || ldx < n) {
+        set_error("bad operand");
+        return RLG_E_INVALID_ARG;
+    }
+    Device *d;
+    RLG_TRY(current_device(&d));
+    hipStream_t s = d->stream;
+    PoolBuf dm, dx;
+    RLG_TRY(dm.alloc(n * n * sizeof(T), s));
+    RLG_TRY(dx.alloc(nrhs * n * sizeof(T), s));
+    RLG_TRY(upload_matrix<T>(s, dm.as<T>(), n, m, ldm, n, n));
+    RLG_TRY(upload_matrix<T>(s, dx.as<T>(), n, b, ldb, nrhs, n));
+    const int st = tri_solve_dev<T>(s, upper, n, dm.as<T>(), n, nrhs, dx.as<T>(), n, dx.as<T>(), n);
+    if (st != RLG_OK) return st;
+    RLG_TRY(download_matrix<T>(s, x, ldx, dx.as<T>(), n, nrhs, n));
+    RLG_HIP(hipStreamSynchronize(s));
+    return RLG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rlg_solve_u_triangular_f64(size_t rows, size_t cols, const double *u, size_t ldu, size_t b_len,
+                               size_t nrhs, const double *b, size_t ldb, double *x, size_t ldx) {
+    return tri_solve_host<double>(true, rows, cols, u, ldu, b_len, nrhs, b, ldb, x, ldx);
+}
+
+int rlg_solve_l_triangular_f64(size_t rows, size_t cols, const double *l, size_t ldl, size_t b_len,
+                               size_t nrhs, const double *b, size_t ldb, double *x, size_t ldx) {
+    return tri_solve_host<double>(false, rows, cols, l, ldl, b_len, nrhs, b, ldb, x, ldx);
+}
+
+int rlg_solve_u_triangular_f32(size_t rows, size_t cols, const float *u, size_t ldu, size_t b_len,
+                               size_t nrhs, const float *b, size_t ldb, float *x, size_t ldx) {
+    return tri_solve_host<float>(true, rows, cols, u, ldu, b_len, nrhs, b, ldb, x, ldx);
+}
+
+int rlg_solve_l_triangular_f32(size_t rows, size_t cols, const float *l, size_t ldl, size_t b_len,
+                               size_t nrhs, const float *b, size_t ldb, float *x, size_t ldx) {
+    return tri_solve_host<float>(false, rows, cols, l, ldl, b_len, nrhs, b, ldb, x, ldx);
+}
+
+int rlg_solve_triangular_f64_dev(int upper, size_t n, const double *m, size_t ldm, size_t nrhs,
+                                 const double *b, size_t ldb, double *x, size_t ldx, void *stream) {
+    if (n == 0 || nrhs == 0) return RLG_OK;
+    if (!m || !b || !x || ldm < n || ldb < n || ldx < n) {
+        set_error("bad operand");
+        return RLG_E_INVALID_ARG;
+    }
+    Device *d;
+    RLG_TRY(current_device(&d));
+    return tri_solve_dev<double>(pick_stream(d, stream), upper != 0, n, m, ldm, nrhs, b, ldb, x, ldx);
 }
 
 // ------------------------------------------------------------------ LU
@@ -513,8 +619,11 @@ int rlg_chol_solve_f64_dev(const rlg_chol *ch, size_t nrhs, const double *b, siz
     if (ldb < ch->n || ldx < ch->n) { set_error("bad right-hand side stride"); return RLG_E_INVALID_ARG; }
     Device *d;
     RLG_TRY(current_device(&d));
-    return chol_solve<double>(pick_stream(d, stream), ch->n, static_cast<const double *>(ch->a), ch->lda,
-                              nrhs, b, ldb, x, ldx);
+    hipStream_t s = pick_stream(d, stream);
+    PoolBuf tmp;  // returned to the pool with an event: the call stays asynchronous
+    RLG_TRY(tmp.alloc(nrhs * ch->n * 8, s));
+    return chol_solve<double>(s, ch->n, static_cast<const double *>(ch->a), ch->lda, nrhs, b, ldb, x,
+                              ldx, tmp.as<double>());
 }
 
 int rlg_chol_size(const rlg_chol *ch, size_t *n) {

@@ -170,20 +170,24 @@ int chol_factor_lowp(hipStream_t s, int prec, size_t n, float *a, size_t lda, in
     return chol_impl<float>(s, n, a, lda, info, work, prec);
 }
 
+// Cholesky::solve (cholesky.rs:194-202): forward substitution with L, then
+// transpose_back_substitution with L^T.  b is copied to tmp first, so x may
+// alias b; the first solve writes x, the second reads x and writes tmp.
 template <class T>
 int chol_solve(hipStream_t s, size_t n, const T *l, size_t lda, size_t nrhs, const T *b, size_t ldb,
-               T *x, size_t ldx) {
+               T *x, size_t ldx, T *tmp) {
     if (n == 0 || nrhs == 0) return RLG_OK;
-    if (x != b)
-        for (size_t r = 0; r < nrhs; ++r)
-            RLG_HIP(hipMemcpyAsync(x + r * ldx, b + r * ldb, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-    RLG_TRY(trsv_lower<T>(s, n, l, lda, false, nrhs, x, ldx));
-    RLG_TRY(trsv_lower_t<T>(s, n, l, lda, nrhs, x, ldx));
+    RLG_HIP(hipMemcpy2DAsync(tmp, n * sizeof(T), b, ldb * sizeof(T), n * sizeof(T), nrhs,
+                             hipMemcpyDeviceToDevice, s));
+    RLG_TRY(trsv_lower<T>(s, n, l, lda, false, nrhs, tmp, n, x, ldx));
+    RLG_TRY(trsv_lower_t<T>(s, n, l, lda, nrhs, x, ldx, tmp, n));
+    RLG_HIP(hipMemcpy2DAsync(x, ldx * sizeof(T), tmp, n * sizeof(T), n * sizeof(T), nrhs,
+                             hipMemcpyDeviceToDevice, s));
     return RLG_OK;
 }
 template int chol_solve<double>(hipStream_t, size_t, const double *, size_t, size_t, const double *,
-                                size_t, double *, size_t);
+                                size_t, double *, size_t, double *);
 template int chol_solve<float>(hipStream_t, size_t, const float *, size_t, size_t, const float *,
-                               size_t, float *, size_t);
+                               size_t, float *, size_t, float *);
 
 }  // namespace rlg

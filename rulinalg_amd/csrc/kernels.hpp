@@ -88,23 +88,31 @@ int chol_factor(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info,
                 void *work);
 int chol_factor_lowp(hipStream_t s, int prec, size_t n, float *a, size_t lda,
                      int64_t *info, void *work);
+// X = L^-T L^-1 B for nrhs vectors; x may alias b; `tmp` holds n * nrhs
+// elements.
 template <class T>
 int chol_solve(hipStream_t s, size_t n, const T *l, size_t lda, size_t nrhs,
-               const T *b, size_t ldb, T *x, size_t ldx);
+               const T *b, size_t ldb, T *x, size_t ldx, T *tmp);
 
 // ------------------------------------------------------------------ triangular
-// In-place solves on nrhs vectors (vector r at x + r*ldx):
-//   lower:  L y = x   (unit: diagonal taken as 1)
-//   upper:  U y = x
-//   lower_t: L^T y = x
+// Solves on nrhs vectors: the right-hand sides come in `in` (vector r at
+// in + r*ldi, overwritten as the running residual) and the solution goes to
+// `out` (vector r at out + r*ldo); `in` and `out` must not overlap.
+//   lower:  L y = b   (unit: diagonal taken as 1)
+//   upper:  U y = b
+//   lower_t: L^T y = b
 template <class T>
-int trsv_lower(hipStream_t s, size_t n, const T *l, size_t lda, bool unit,
-               size_t nrhs, T *x, size_t ldx);
+int trsv_lower(hipStream_t s, size_t n, const T *l, size_t lda, bool unit, size_t nrhs,
+               T *in, size_t ldi, T *out, size_t ldo);
 template <class T>
-int trsv_upper(hipStream_t s, size_t n, const T *u, size_t lda, size_t nrhs,
-               T *x, size_t ldx);
+int trsv_upper(hipStream_t s, size_t n, const T *u, size_t lda, size_t nrhs, T *in,
+               size_t ldi, T *out, size_t ldo);
 template <class T>
-int trsv_lower_t(hipStream_t s, size_t n, const T *l, size_t lda, size_t nrhs,
-                 T *x, size_t ldx);
+int trsv_lower_t(hipStream_t s, size_t n, const T *l, size_t lda, size_t nrhs, T *in,
+                 size_t ldi, T *out, size_t ldo);
+// *flag = 1 when some |m_ii| < eps (left untouched otherwise; the caller
+// clears it first).
+template <class T>
+int diag_tiny(hipStream_t s, size_t n, const T *m, size_t lda, T eps, uint32_t *flag);
 
 }  // namespace rlg

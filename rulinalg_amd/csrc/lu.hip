@@ -375,7 +375,8 @@ static int panel(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, 
         set_error("LU panel of %zu rows needs %u workgroups (max %d)", m, G, GMAX);
         return RLG_E_INVALID_ARG;
     }
-    RLG_PROF("lu_panel", s);
+    // algorithmic bytes: the m x w panel read once and written once
+    RLG_PROF_W("lu_panel", s, 2.0 * (double)m * (double)w * sizeof(T));
     hipLaunchKernelGGL(k_lu_panel<T>, dim3(G), dim3(PTHREADS), 0, s, a, (long)lda, (int)k, (int)m,
                        (int)w, ipiv, info, (int)n, bar, static_cast<T *>(lw.cval), lw.crow,
                        static_cast<T *>(lw.cdat), static_cast<T *>(lw.rowj), lw.timeout);
@@ -388,7 +389,8 @@ static int laswp(hipStream_t s, T *a, size_t lda, const int64_t *ipiv, size_t k0
                  size_t c0, size_t c1, size_t c2, size_t c3, const LuWork &lw) {
     const size_t ncols = (c1 - c0) + (c3 - c2);
     if (ncols == 0 || cnt == 0) return RLG_OK;
-    RLG_PROF("laswp", s);
+    // algorithmic bytes: at most 2 cnt rows change, each read and written
+    RLG_PROF_W("laswp", s, 2.0 * 2.0 * (double)cnt * (double)ncols * sizeof(T));
     hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, s, ipiv, (int)k0, (int)cnt, lw.map);
     RLG_LAUNCHED();
     hipLaunchKernelGGL(k_laswp<T>, dim3(ceil_div(ncols, LCOLS)), dim3(256), 0, s, a, (long)lda,
@@ -400,7 +402,8 @@ static int laswp(hipStream_t s, T *a, size_t lda, const int64_t *ipiv, size_t k0
 template <class T>
 static int trsm_llu(hipStream_t s, T *a, size_t lda, size_t j0, size_t jb, size_t c0, size_t c1) {
     if (c1 <= c0 || jb <= 1) return RLG_OK;
-    RLG_PROF("trsm", s);
+    // algorithmic bytes: the unit-lower jb x jb block and the jb x (c1-c0) block row (read + write)
+    RLG_PROF_W("trsm", s, ((double)jb * jb / 2 + 2.0 * (double)jb * (double)(c1 - c0)) * sizeof(T));
     hipLaunchKernelGGL(k_trsm_llu<T>, dim3(ceil_div(c1 - c0, 64)), dim3(256), 0, s, a, (long)lda,
                        (int)j0, (int)jb, (int)c0, (int)c1);
     RLG_LAUNCHED();
@@ -554,10 +557,11 @@ int lu_solve(hipStream_t s, size_t n, const T *lu, size_t lda, const int64_t *ra
                            nrhs, raw, b, ldb, tmp, n);
         RLG_LAUNCHED();
     }
-    RLG_TRY(trsv_lower<T>(s, n, lu, lda, true, nrhs, tmp, n));
-    RLG_TRY(trsv_upper<T>(s, n, lu, lda, nrhs, tmp, n));
-    for (size_t r = 0; r < nrhs; ++r)
-        RLG_HIP(hipMemcpyAsync(x + r * ldx, tmp + r * n, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    // y = L^-1 (P b) lands in x (b is no longer read), then U^-1 y in tmp
+    RLG_TRY(trsv_lower<T>(s, n, lu, lda, true, nrhs, tmp, n, x, ldx));
+    RLG_TRY(trsv_upper<T>(s, n, lu, lda, nrhs, x, ldx, tmp, n));
+    RLG_HIP(hipMemcpy2DAsync(x, ldx * sizeof(T), tmp, n * sizeof(T), n * sizeof(T), nrhs,
+                             hipMemcpyDeviceToDevice, s));
     return RLG_OK;
 }
 template int lu_solve<double>(hipStream_t, size_t, const double *, size_t, const int64_t *, size_t,

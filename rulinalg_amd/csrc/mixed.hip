@@ -179,7 +179,8 @@ static int residual(hipStream_t s, size_t n, const double *a, size_t lda, size_t
                     const double *x, size_t ldx, const double *b, size_t ldb, double *res, size_t ldr,
                     double *rmax) {
     RLG_TRY(zero(s, rmax, nrhs));
-    RLG_PROF("residual_f64", s);
+    // algorithmic bytes: A once, plus x and b read and res written per RHS
+    RLG_PROF_W("residual_f64", s, 8.0 * ((double)n * (double)n + 3.0 * (double)n * (double)nrhs));
     for (size_t r0 = 0; r0 < nrhs; r0 += 8) {
         const int nr = (int)std::min<size_t>(8, nrhs - r0);
         hipLaunchKernelGGL(k_residual<8>, dim3(ceil_div(n, 4)), dim3(256), 0, s, (int)n, a, (long)lda,

@@ -288,7 +288,7 @@ struct rlg_server {
                                          sl.b.as<double>(), n, sl.x.as<double>(), n, sl.tmp.as<double>()));
             else
                 RLG_TRY(chol_solve<double>(s, n, f.f.as<double>(), n, nr, sl.b.as<double>(), n,
-                                           sl.x.as<double>(), n));
+                                           sl.x.as<double>(), n, sl.tmp.as<double>()));
             RLG_TRY(backward_error_enqueue(s, n, f.a.as<double>(), n, nr, sl.b.as<double>(), n,
                                            sl.x.as<double>(), n, sl.norms.as<double>(),
                                            sl.res.as<double>(), sl.rs.as<double>()));
