@@ -286,6 +286,190 @@ __global__ __launch_bounds__(GTHREADS) void k_gemm_lowp(
             }
 }
 
+// ---------------------------------------------------------------- packed low precision
+//
+// The packed form rounds each operand to FP16/BF16 once, into k-contiguous
+// rows (A as m x Kp, B as n x Kp, with Kp = K rounded up to 64 and the pad
+// zero), so the GEMM itself streams 16-byte chunks straight into LDS and
+// needs no conversion or transposition in its inner loop.
+
+constexpr int PKB = 64;  // k granularity of the packed operands
+
+// dst[i][k] = src[i][k] (k < K), 0 for K <= k < Kp.  One thread per 8 k.
+template <class H>
+__global__ __launch_bounds__(256) void k_pack_rows(const float *__restrict__ src, long lds, int rows,
+                                                   int K, int Kp, H *__restrict__ dst, long ldd) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    const int chunks = Kp / 8;
+    const long i = e / chunks;
+    const int k0 = (int)(e - i * chunks) * 8;
+    if (i >= rows) return;
+    const float *s = src + i * lds + k0;
+    typedef H h8v __attribute__((ext_vector_type(8)));
+    h8v v;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = LowpMfma<H>::cvt(k0 + q < K ? s[q] : 0.f);
+    *reinterpret_cast<h8v *>(dst + i * ldd + k0) = v;
+}
+
+// dst[j][k] = src[k][j] for a K x cols source (k < K), 0 for K <= k < Kp.
+// 64 x 64 tiles through LDS: coalesced reads along j, writes along k.
+template <class H>
+__global__ __launch_bounds__(256) void k_pack_cols(const float *__restrict__ src, long lds, int K,
+                                                   int cols, int Kp, H *__restrict__ dst, long ldd) {
+    __shared__ float t[64][65];
+    const int k0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int kk = e >> 6, jj = e & 63;
+        const int k = k0 + kk, j = j0 + jj;
+        t[kk][jj] = (k < K && j < cols) ? src[(long)k * lds + j] : 0.f;
+    }
+    __syncthreads();
+    typedef H h8v __attribute__((ext_vector_type(8)));
+    for (int e = threadIdx.x; e < 64 * 8; e += 256) {
+        const int jj = e >> 3, kc = (e & 7) * 8;
+        const int j = j0 + jj;
+        if (j >= cols || k0 + kc >= Kp) continue;
+        h8v v;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = LowpMfma<H>::cvt(t[kc + q][jj]);
+        *reinterpret_cast<h8v *>(dst + (long)j * ldd + k0 + kc) = v;
+    }
+}
+
+// C = alpha * Ah * Bh^T + beta * C with Ah (M x Kp) and Bh (N x Kp) packed,
+// Kp a multiple of 64.  128 x 128 tile per workgroup, 2 x 2 waves of 64 x 64,
+// k staged 64 at a time through double-buffered LDS with 16-byte moves.
+template <class H, bool LOWER>
+__global__ __launch_bounds__(GTHREADS) void k_gemm_nt_lowp(
+    int M, int N, int Kp, float alpha, const H *__restrict__ A, long lda, const H *__restrict__ B,
+    long ldb, float beta, float *__restrict__ C, long ldc, int tiles_m, int tiles_n) {
+    using V = typename LowpMfma<H>::vec;
+    __shared__ __attribute__((aligned(16))) H As[2][GBM][LBK + LPAD];
+    __shared__ __attribute__((aligned(16))) H Bs[2][GBN][LBK + LPAD];
+
+    int tm, tn;
+    tile_coords(tiles_m, tiles_n, tm, tn);
+    if (LOWER && tn > tm) return;
+    const int m0 = tm * GBM, n0 = tn * GBN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4{0, 0, 0, 0};
+
+    // a stage is 128 rows x 64 k per operand = 1024 chunks of 8 halves: 4 per thread
+    uint4 ra[4], rb[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = q * GTHREADS + tid;
+            const int row = c >> 3, kc = (c & 7) * 8;
+            const int gi = m0 + row, gj = n0 + row;
+            ra[q] = gi < M ? *reinterpret_cast<const uint4 *>(A + (long)gi * lda + k0 + kc)
+                           : uint4{0, 0, 0, 0};
+            rb[q] = gj < N ? *reinterpret_cast<const uint4 *>(B + (long)gj * ldb + k0 + kc)
+                           : uint4{0, 0, 0, 0};
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = q * GTHREADS + tid;
+            const int row = c >> 3, kc = (c & 7) * 8;
+            *reinterpret_cast<uint4 *>(&As[buf][row][kc]) = ra[q];
+            *reinterpret_cast<uint4 *>(&Bs[buf][row][kc]) = rb[q];
+        }
+    };
+
+    const int KT = Kp / LBK;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < KT) load((kt + 1) * LBK);
+#pragma unroll
+        for (int ks = 0; ks < LBK / 32; ++ks) {
+            V af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                af[i] = *reinterpret_cast<const V *>(&As[buf][wm + i * 16 + lr][ks * 32 + lk * 8]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                bf[j] = *reinterpret_cast<const V *>(&Bs[buf][wn + j * 16 + lr][ks * 32 + lk * 8]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = LowpMfma<H>::op(af[i], bf[j], acc[i][j]);
+        }
+        if (kt + 1 < KT) store(buf ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wm + i * 16 + lk * 4 + r;
+                const int gj = n0 + wn + j * 16 + lr;
+                if (gi < M && gj < N) {
+                    float *p = C + (long)gi * ldc + gj;
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.f) v += beta * *p;
+                    *p = v;
+                }
+            }
+}
+
+size_t gemm_lowp_pack_bytes(size_t m, size_t n, size_t k) {
+    const size_t kp = (k + PKB - 1) / PKB * PKB;
+    return ((m * kp * 2 + 255) & ~(size_t)255) + ((n * kp * 2 + 255) & ~(size_t)255);
+}
+
+template <class H>
+static int gemm_lowp_packed(hipStream_t s, const char *cls, size_t m, size_t n, size_t k, float alpha,
+                            const float *a, size_t lda, const float *b, size_t ldb, float beta,
+                            float *c, size_t ldc, bool trans_b, bool lower_only, void *pack) {
+    const size_t kp = (k + PKB - 1) / PKB * PKB;
+    H *ah = static_cast<H *>(pack);
+    H *bh = reinterpret_cast<H *>(static_cast<char *>(pack) + ((m * kp * 2 + 255) & ~(size_t)255));
+    {
+        RLG_PROF_W("pack_lowp", s, 4.0 * (double)(m + n) * k + 2.0 * (double)(m + n) * kp);
+        hipLaunchKernelGGL(k_pack_rows<H>, dim3(ceil_div(m * (kp / 8), 256)), dim3(256), 0, s, a,
+                           (long)lda, (int)m, (int)k, (int)kp, ah, (long)kp);
+        RLG_LAUNCHED();
+        if (trans_b)
+            hipLaunchKernelGGL(k_pack_rows<H>, dim3(ceil_div(n * (kp / 8), 256)), dim3(256), 0, s, b,
+                               (long)ldb, (int)n, (int)k, (int)kp, bh, (long)kp);
+        else
+            hipLaunchKernelGGL(k_pack_cols<H>, dim3(ceil_div(n, 64), ceil_div(kp, 64)), dim3(256), 0,
+                               s, b, (long)ldb, (int)k, (int)n, (int)kp, bh, (long)kp);
+        RLG_LAUNCHED();
+    }
+    const int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
+    const double flops = (lower_only ? 1.0 : 2.0) * (double)m * (double)n * (double)k;
+    static const int cls_id = prof_class(cls);
+    ProfScope prof_scope(cls_id, s, flops);
+    if (lower_only)
+        hipLaunchKernelGGL((k_gemm_nt_lowp<H, true>), dim3(tiles_m * tiles_n), dim3(GTHREADS), 0, s,
+                           (int)m, (int)n, (int)kp, alpha, ah, (long)kp, bh, (long)kp, beta, c,
+                           (long)ldc, tiles_m, tiles_n);
+    else
+        hipLaunchKernelGGL((k_gemm_nt_lowp<H, false>), dim3(tiles_m * tiles_n), dim3(GTHREADS), 0, s,
+                           (int)m, (int)n, (int)kp, alpha, ah, (long)kp, bh, (long)kp, beta, c,
+                           (long)ldc, tiles_m, tiles_n);
+    RLG_LAUNCHED();
+    return RLG_OK;
+}
+
 // ---------------------------------------------------------------- launchers
 
 static int check_dims(size_t m, size_t n, size_t k) {
@@ -336,9 +520,17 @@ int gemm_f32(hipStream_t s, size_t m, size_t n, size_t k, float alpha, const flo
 
 int gemm_lowp(hipStream_t s, int prec, size_t m, size_t n, size_t k, float alpha,
               const float *a, size_t lda, const float *b, size_t ldb, float beta, float *c,
-              size_t ldc, bool trans_b, bool lower_only) {
+              size_t ldc, bool trans_b, bool lower_only, void *pack, size_t pack_bytes) {
     if (m == 0 || n == 0) return RLG_OK;
     RLG_TRY(check_dims(m, n, k));
+    if (k > 0 && pack && pack_bytes >= gemm_lowp_pack_bytes(m, n, k)) {
+        if (prec == RLG_PREC_F16)
+            return gemm_lowp_packed<_Float16>(s, "gemm_f16", m, n, k, alpha, a, lda, b, ldb, beta, c,
+                                              ldc, trans_b, lower_only, pack);
+        if (prec == RLG_PREC_BF16)
+            return gemm_lowp_packed<__bf16>(s, "gemm_bf16", m, n, k, alpha, a, lda, b, ldb, beta, c,
+                                            ldc, trans_b, lower_only, pack);
+    }
     const int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
     dim3 grid(tiles_m * tiles_n), block(GTHREADS);
 #define RLG_LOWP_LAUNCH(H, TB, LO)                                                           \

@@ -22,10 +22,16 @@ int gemm_f32(hipStream_t s, size_t m, size_t n, size_t k, float alpha,
              bool lower_only = false);
 // FP32 operands rounded to FP16 (prec = RLG_PREC_F16) or BF16
 // (RLG_PREC_BF16) on their way into LDS, products accumulated in FP32.
+// With `pack` (at least gemm_lowp_pack_bytes(m, n, k) bytes of device
+// memory) both operands are first rounded into k-contiguous packed copies
+// and a plain NT kernel runs on them; without it the rounding happens on the
+// way into LDS.
 int gemm_lowp(hipStream_t s, int prec, size_t m, size_t n, size_t k,
               float alpha, const float *a, size_t lda, const float *b,
               size_t ldb, float beta, float *c, size_t ldc,
-              bool trans_b = false, bool lower_only = false);
+              bool trans_b = false, bool lower_only = false,
+              void *pack = nullptr, size_t pack_bytes = 0);
+size_t gemm_lowp_pack_bytes(size_t m, size_t n, size_t k);
 
 template <class T>
 inline int gemm_t(hipStream_t s, size_t m, size_t n, size_t k, T alpha,
@@ -62,6 +68,9 @@ int lu_factor(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv,
               int64_t *info, void *work);
 // Mixed-precision variant of the FP32 factorization whose trailing updates
 // run on FP16/BF16 MFMA (prec = RLG_PREC_F16 / RLG_PREC_BF16).
+// Its workspace is lu_lowp_work_bytes(n) bytes (lu_work_bytes plus room for
+// the packed low-precision operands of the trailing update).
+size_t lu_lowp_work_bytes(size_t n);
 int lu_factor_lowp(hipStream_t s, int prec, size_t n, float *a, size_t lda,
                    int64_t *ipiv, int64_t *info, void *work);
 

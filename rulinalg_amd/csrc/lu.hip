@@ -421,10 +421,14 @@ static int update_native(hipStream_t s, size_t m, size_t n, size_t k, const T *a
     return gemm_t<T>(s, m, n, k, T(-1), a, lda, b, ldb, T(1), c, ldc);
 }
 
-static thread_local int g_lowp_prec = RLG_PREC_F16;  // set by lu_factor_lowp for its duration
+// set by lu_factor_lowp for its duration: precision and the packed-operand area
+static thread_local int g_lowp_prec = RLG_PREC_F16;
+static thread_local void *g_lowp_pack = nullptr;
+static thread_local size_t g_lowp_pack_bytes = 0;
 static int update_lowp(hipStream_t s, size_t m, size_t n, size_t k, const float *a, size_t lda,
                        const float *b, size_t ldb, float *c, size_t ldc) {
-    return gemm_lowp(s, g_lowp_prec, m, n, k, -1.f, a, lda, b, ldb, 1.f, c, ldc);
+    return gemm_lowp(s, g_lowp_prec, m, n, k, -1.f, a, lda, b, ldb, 1.f, c, ldc, false, false,
+                     g_lowp_pack, g_lowp_pack_bytes);
 }
 
 // Events used to order the look-ahead stream against the main stream.
@@ -466,6 +470,8 @@ static int outer_panel(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv,
 // LOOK_AHEAD is off: a run with it on stalled on the GPU and the stall has
 // not been traced to its cause, so every update runs on the main stream.
 constexpr bool LOOK_AHEAD = false;
+// The packed low-precision operand area serves one trailing update at a time.
+static_assert(!LOOK_AHEAD, "concurrent trailing updates would share the packed operand area");
 template <class T>
 static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv, int64_t *info,
                           void *work, UpdateFn<T> big_update) {
@@ -528,10 +534,20 @@ int lu_factor(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv, int64_t 
 template int lu_factor<double>(hipStream_t, size_t, double *, size_t, int64_t *, int64_t *, void *);
 template int lu_factor<float>(hipStream_t, size_t, float *, size_t, int64_t *, int64_t *, void *);
 
+// The trailing updates are at most n x n with depth ONB.
+size_t lu_lowp_work_bytes(size_t n) {
+    return align_up(lu_work_bytes(n, 4)) + gemm_lowp_pack_bytes(n, n, ONB);
+}
+
 int lu_factor_lowp(hipStream_t s, int prec, size_t n, float *a, size_t lda, int64_t *ipiv,
                    int64_t *info, void *work) {
     g_lowp_prec = prec;
-    return lu_factor_impl<float>(s, n, a, lda, ipiv, info, work, &update_lowp);
+    g_lowp_pack = static_cast<char *>(work) + align_up(lu_work_bytes(n, 4));
+    g_lowp_pack_bytes = gemm_lowp_pack_bytes(n, n, ONB);
+    const int st = lu_factor_impl<float>(s, n, a, lda, ipiv, info, work, &update_lowp);
+    g_lowp_pack = nullptr;
+    g_lowp_pack_bytes = 0;
+    return st;
 }
 
 // ---------------------------------------------------------------- solve

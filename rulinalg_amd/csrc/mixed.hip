@@ -292,7 +292,7 @@ int solve_mixed_dev(hipStream_t s, size_t n, const double *a, size_t lda, size_t
         RLG_TRY(lu32.alloc(n * n * 4, s));
         RLG_TRY(ipiv.alloc(n * 8, s));
         RLG_TRY(info.alloc(8, s));
-        RLG_TRY(work.alloc(lu_work_bytes(n, 4), s));
+        RLG_TRY(work.alloc(o.factor_prec == RLG_PREC_F32 ? lu_work_bytes(n, 4) : lu_lowp_work_bytes(n), s));
         RLG_TRY(t32.alloc(nrhs * n * 4, s));
         RLG_TRY(tmp32.alloc(nrhs * n * 4, s));
         int64_t info_h = 0;

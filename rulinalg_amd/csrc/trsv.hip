@@ -162,13 +162,14 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
             }
 #pragma unroll
             for (int r = 0; r < TMAXR; ++r) {
-                if (r >= nrhs) break;
-                T v = acc[r];
-                v += __shfl_xor(v, 8, 64);
-                v += __shfl_xor(v, 4, 64);
-                v += __shfl_xor(v, 2, 64);
-                v += __shfl_xor(v, 1, 64);
-                if (act && ls == 0) W[(long)r * ldw + j] -= v;
+                if (r < nrhs) {  // wave-uniform
+                    T v = acc[r];
+                    v += __shfl_xor(v, 8, 64);
+                    v += __shfl_xor(v, 4, 64);
+                    v += __shfl_xor(v, 2, 64);
+                    v += __shfl_xor(v, 1, 64);
+                    if (act && ls == 0) W[(long)r * ldw + j] -= v;
+                }
             }
         }
     }

@@ -408,7 +408,10 @@ def test_mixed_refined_well_conditioned(prec):
     assert rep["converged"] == 1
     for i in range(2):
         assert eta(a, x[i], b[i]) <= 1e-13
-    if prec == _lib.PREC_F32:
+    # kappa * u is 1e-5 (FP32) and 0.05 (FP16): refinement must converge
+    # without the FP64 fallback, which also shows the low-precision trailing
+    # updates (orders 600 = 256 + 256 + 88 use two of them) are right
+    if prec != _lib.PREC_BF16:
         assert rep["fell_back"] == 0
 
 

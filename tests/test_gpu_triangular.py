@@ -130,16 +130,22 @@ def test_triangular_f32():
 
 
 def test_triangular_dev_in_place(lib):
-    import torch
     n, r = 700, 3
     t, full = tri(n, True, 21)
     b = wl.rng(22).standard_normal((r, n))
-    dm = torch.from_numpy(full).cuda()
-    dx = torch.from_numpy(b).cuda()
-    torch.cuda.synchronize()
-    st = lib.rlg_solve_triangular_f64_dev(1, n, dm.data_ptr(), n, r, dx.data_ptr(), n,
-                                          dx.data_ptr(), n, None)
-    assert st == 0, _lib.last_error()
-    x = dx.cpu().numpy()
+    full = np.ascontiguousarray(full)
+    dm, dx = C.c_void_p(), C.c_void_p()
+    assert lib.rlg_dev_alloc(full.nbytes, C.byref(dm)) == 0
+    assert lib.rlg_dev_alloc(b.nbytes, C.byref(dx)) == 0
+    try:
+        assert lib.rlg_dev_upload(dm, full.ctypes.data, full.nbytes) == 0
+        assert lib.rlg_dev_upload(dx, b.ctypes.data, b.nbytes) == 0
+        st = lib.rlg_solve_triangular_f64_dev(1, n, dm, n, r, dx, n, dx, n, None)
+        assert st == 0, _lib.last_error()
+        x = np.empty_like(b)
+        assert lib.rlg_dev_download(x.ctypes.data, dx, x.nbytes) == 0
+    finally:
+        lib.rlg_dev_free(dm)
+        lib.rlg_dev_free(dx)
     for i in range(r):
         assert eta(t, x[i], b[i]) <= 1e-13
