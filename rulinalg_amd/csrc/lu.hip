@@ -144,9 +144,19 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
     const int rows = min(PR, m - r0);
     const int lane = tid & 63, wave = tid >> 6;
 
-    for (int e = tid; e < rows * w; e += PTHREADS) {
-        const int r = e / w, c = e - r * w;
-        tile[r][c] = A[(long)(k + r0 + r) * lda + k + c];
+    {   // all loads in flight before the LDS stores (PR x PNB / PTHREADS per thread)
+        constexpr int PER = PR * PNB / PTHREADS;
+        T v[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = q * PTHREADS + tid, r = e / PNB, c = e % PNB;
+            v[q] = (r < rows && c < w) ? A[(long)(k + r0 + r) * lda + k + c] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = q * PTHREADS + tid, r = e / PNB, c = e % PNB;
+            if (r < rows && c < w) tile[r][c] = v[q];
+        }
     }
     __syncthreads();
 
@@ -260,14 +270,18 @@ __global__ __launch_bounds__(64) void k_swap_map(const int64_t *__restrict__ ipi
                                                  int *__restrict__ map) {
     __shared__ int content[ONB];          // content[r - k0] for r in [k0, k0+cnt)
     __shared__ int ext_row[ONB], ext_src[ONB];
+    __shared__ int piv[ONB];              // the step pivots, loaded once
     __shared__ int n_ext;
     const int lane = threadIdx.x;
-    for (int i = lane; i < cnt; i += 64) content[i] = k0 + i;
+    for (int i = lane; i < cnt; i += 64) {
+        content[i] = k0 + i;
+        piv[i] = (int)ipiv[k0 + i];
+    }
     if (lane == 0) n_ext = 0;
     __syncthreads();
     for (int s = 0; s < cnt; ++s) {
         const int r = k0 + s;
-        const int p = (int)ipiv[k0 + s];
+        const int p = piv[s];
         if (p == r) continue;
         if (p < k0 + cnt) {
             if (lane == 0) { int t = content[s]; content[s] = content[p - k0]; content[p - k0] = t; }
@@ -319,8 +333,21 @@ __global__ __launch_bounds__(256) void k_laswp(T *__restrict__ A, long lda, cons
     const int lc = cb + cl;
     const int col = lc < w1 ? c0 + lc : c2 + (lc - w1);
     const bool okc = lc < w1 + (c3 - c2);
-    for (int i = rg; i < cnt; i += 256 / LCOLS)
-        if (okc) stage[i][cl] = A[(long)src[i] * lda + col];
+    // eight loads in flight per thread before their LDS stores
+    constexpr int RG = 256 / LCOLS;
+    for (int i0 = rg; i0 < cnt; i0 += 8 * RG) {
+        T v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = i0 + q * RG;
+            v[q] = (okc && i < cnt) ? A[(long)src[i] * lda + col] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = i0 + q * RG;
+            if (i < cnt) stage[i][cl] = v[q];
+        }
+    }
     __syncthreads();
     for (int i = rg; i < cnt; i += 256 / LCOLS)
         if (okc) A[(long)dst[i] * lda + col] = stage[i][cl];
@@ -333,6 +360,9 @@ __global__ __launch_bounds__(256) void k_laswp(T *__restrict__ A, long lda, cons
 // for each step c, b[r] = b[r] - l[r][c] * b[c] for r > c.  This is the
 // order in which gaussian_elimination (lu.rs:603-616) updates the rows of
 // the block, with the same separately rounded multiply and subtract.
+//
+// L and the 64-column slab of B are staged in LDS with every load issued
+// before the stores; four waves split the rows of each sweep step.
 template <class T>
 __global__ __launch_bounds__(256) void k_trsm_llu(T *__restrict__ A, long lda, int j0, int jb, int c0,
                                                   int c1) {
@@ -341,13 +371,21 @@ __global__ __launch_bounds__(256) void k_trsm_llu(T *__restrict__ A, long lda, i
     const int tid = threadIdx.x;
     const int cb = c0 + blockIdx.x * 64;
     const int nc = min(64, c1 - cb);
-    for (int e = tid; e < jb * jb; e += 256) {
-        const int r = e / jb, c = e - r * jb;
-        L[r][c] = A[(long)(j0 + r) * lda + j0 + c];
-    }
-    for (int e = tid; e < jb * 64; e += 256) {
-        const int r = e >> 6, c = e & 63;
-        X[r][c] = c < nc ? A[(long)(j0 + r) * lda + cb + c] : T(0);
+    {
+        constexpr int PER = PNB * PNB / 256;
+        T vl[PER], vx[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = q * 256 + tid, r = e >> 6, c = e & 63;
+            vl[q] = (r < jb && c < jb) ? A[(long)(j0 + r) * lda + j0 + c] : T(0);
+            vx[q] = (r < jb && c < nc) ? A[(long)(j0 + r) * lda + cb + c] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = q * 256 + tid, r = e >> 6, c = e & 63;
+            L[r][c] = vl[q];
+            X[r][c] = vx[q];
+        }
     }
     __syncthreads();
     const int cj = tid & 63, rq = tid >> 6;

@@ -32,7 +32,7 @@ constexpr int SB = 64;      // sub-block rows (the lanes of one wave)
 constexpr int TSB = 2;      // sub-blocks per step
 constexpr int TMAXR = 16;   // right-hand sides per launch
 constexpr int RPW = 64;     // rows updated per workgroup (row-access forms)
-constexpr int CPW = 256;    // entries updated per workgroup (transposed form)
+constexpr int CPW = 64;     // entries updated per workgroup (transposed form)
 
 enum TriMode { LOWER_UNIT = 0, LOWER = 1, UPPER = 2, LOWER_T = 3 };
 
@@ -44,14 +44,25 @@ template <int MODE> __device__ __forceinline__ constexpr bool stored_upper() { r
 // Stage one 64 x 64 tile of M (rows rb.., cols cb.., rs x cs valid) into
 // LDS; `diag` keeps only the stored triangle and zeroes the rest, so entries
 // of the other triangle are never read as numbers.
+// All 16 loads of a thread are issued before the first LDS store, so the
+// tile costs one memory latency rather than sixteen.
 template <class T, int MODE>
 __device__ __forceinline__ void stage_tile(T (*tl)[SB + 1], const T *__restrict__ M, long lda,
                                            int rb, int cb, int rs, int cs, bool diag) {
-    for (int e = threadIdx.x; e < SB * SB; e += 256) {
+    constexpr int PER = SB * SB / 256;
+    T v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = q * 256 + threadIdx.x;
         const int i = e >> 6, j = e & 63;
         bool ok = i < rs && j < cs;
         if (diag) ok = ok && (stored_upper<MODE>() ? j >= i : j <= i);
-        tl[i][j] = ok ? M[(long)(rb + i) * lda + cb + j] : T(0);
+        v[q] = ok ? M[(long)(rb + i) * lda + cb + j] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = q * 256 + threadIdx.x;
+        tl[e >> 6][e & 63] = v[q];
     }
 }
 
@@ -123,52 +134,76 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
         }
 
     // ---- 2. remove the solved rows from this workgroup's share of [u0, u1)
+    // Every thread issues all of its matrix loads before the first use, so a
+    // workgroup pays one memory latency per phase, not one per row.
     if (MODE == LOWER_T) {
-        // W[r][j] -= sum_c L[s0 + c][j] * x[c]  (threads own j: coalesced rows of L)
-        const int j = u0 + (int)blockIdx.x * CPW + tid;
-        if (j < u1) {
-            T acc[TMAXR];
+        // W[r][j] -= sum_c L[s0 + c][j] * x[c]: a workgroup owns 64 entries j
+        // (one per lane: coalesced rows of L), wave w sums c in [32w, 32w + 32)
+        constexpr int CW = TSB * SB / 4;
+        const int j = u0 + (int)blockIdx.x * CPW + lane;
+        const bool act = j < u1;
+        T mv[CW];
 #pragma unroll
-            for (int r = 0; r < TMAXR; ++r) acc[r] = T(0);
-            for (int c = 0; c < S; ++c) {
-                const T m = M[(long)(s0 + c) * lda + j];
+        for (int q = 0; q < CW; ++q) {
+            const int c = wave * CW + q;
+            mv[q] = (act && c < S) ? M[(long)(s0 + c) * lda + j] : T(0);
+        }
+        // partial sums meet in LDS (the staged tiles are no longer read)
+        T(*red)[TMAXR][SB] = reinterpret_cast<T(*)[TMAXR][SB]>(&tl[0][0][0]);
+        for (int r = 0; r < nrhs; ++r) {
+            T a = T(0);
 #pragma unroll
-                for (int r = 0; r < TMAXR; ++r)
-                    if (r < nrhs) acc[r] += m * xs[r][c];
+            for (int q = 0; q < CW; ++q) a += mv[q] * xs[r][wave * CW + q];
+            red[wave][r][lane] = a;
+        }
+        __syncthreads();
+        if (wave == 0 && act) {
+            for (int r = 0; r < nrhs; ++r) {
+                const T a = (red[0][r][lane] + red[1][r][lane]) + (red[2][r][lane] + red[3][r][lane]);
+                W[(long)r * ldw + j] -= a;
             }
-#pragma unroll
-            for (int r = 0; r < TMAXR; ++r)
-                if (r < nrhs) W[(long)r * ldw + j] -= acc[r];
         }
     } else {
         // W[r][j] -= sum_c M[j][s0 + c] * x[c]; 16 lanes per row, 4 rows per
         // wave at a time, lane ls reads columns ls, ls + 16, ... (coalesced)
-        const int ls = lane & 15;
-        const int rbase = u0 + (int)blockIdx.x * RPW + wave * (RPW / 4);
-        for (int it = 0; it < RPW / 16; ++it) {
-            const int j = rbase + it * 4 + (lane >> 4);
-            const bool act = j < u1 && j < u0 + ((int)blockIdx.x + 1) * RPW;
-            T acc[TMAXR];
+        constexpr int NIT = RPW / 16, NC = TSB * SB / 16;
+        const int ls = lane & 15, grp = lane >> 4;
+        const int rbase = u0 + (int)blockIdx.x * RPW + wave * (RPW / 4) + grp;
+        T mv[NIT][NC];
 #pragma unroll
-            for (int r = 0; r < TMAXR; ++r) acc[r] = T(0);
-            if (act) {
-                const T *mrow = M + (long)j * lda + s0;
-                for (int c = ls; c < S; c += 16) {
-                    const T m = mrow[c];
+        for (int it = 0; it < NIT; ++it) {
+            const int j = rbase + it * 4;
+            const T *mrow = M + (long)j * lda + s0;
 #pragma unroll
-                    for (int r = 0; r < TMAXR; ++r)
-                        if (r < nrhs) acc[r] += m * xs[r][c];
-                }
+            for (int k = 0; k < NC; ++k) {
+                const int c = ls + 16 * k;
+                mv[it][k] = (j < u1 && c < S) ? mrow[c] : T(0);
             }
+        }
+        for (int r = 0; r < nrhs; ++r) {  // wave-uniform
+            T v[NIT];
 #pragma unroll
-            for (int r = 0; r < TMAXR; ++r) {
-                if (r < nrhs) {  // wave-uniform
-                    T v = acc[r];
-                    v += __shfl_xor(v, 8, 64);
-                    v += __shfl_xor(v, 4, 64);
-                    v += __shfl_xor(v, 2, 64);
-                    v += __shfl_xor(v, 1, 64);
-                    if (act && ls == 0) W[(long)r * ldw + j] -= v;
+            for (int it = 0; it < NIT; ++it) {
+                T a = T(0);
+#pragma unroll
+                for (int k = 0; k < NC; ++k) a += mv[it][k] * xs[r][ls + 16 * k];
+                a += __shfl_xor(a, 8, 64);
+                a += __shfl_xor(a, 4, 64);
+                a += __shfl_xor(a, 2, 64);
+                a += __shfl_xor(a, 1, 64);
+                v[it] = a;
+            }
+            if (ls == 0) {
+                T w[NIT];
+#pragma unroll
+                for (int it = 0; it < NIT; ++it) {
+                    const int j = rbase + it * 4;
+                    w[it] = j < u1 ? W[(long)r * ldw + j] : T(0);
+                }
+#pragma unroll
+                for (int it = 0; it < NIT; ++it) {
+                    const int j = rbase + it * 4;
+                    if (j < u1) W[(long)r * ldw + j] = w[it] - v[it];
                 }
             }
         }
