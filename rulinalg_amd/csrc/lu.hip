@@ -25,7 +25,11 @@
 namespace rlg {
 
 constexpr int PNB = 64;    // inner panel width
-constexpr int PR = 128;    // panel rows per workgroup (LDS-resident)
+constexpr int PR = 128;    // panel rows per workgroup (LDS-resident), FP64
+// Panel rows per workgroup by element type: FP32 panels take 256 rows (the
+// same 66.5 KB of LDS), halving the workgroups that meet at every column.
+template <class T> struct PanelRows { static constexpr int value = PR; };
+template <> struct PanelRows<float> { static constexpr int value = 2 * PR; };
 constexpr int ONB = 256;   // outer block: depth of the trailing GEMM
 constexpr int PTHREADS = 256;
 
@@ -133,19 +137,20 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
                                                        int64_t *__restrict__ info, int n,
                                                        unsigned *bar, T *cval, int *crow,
                                                        T *cdat, T *rowj, int *timeout) {
-    __shared__ T tile[PR][PNB + 1];
+    constexpr int PRT = PanelRows<T>::value;
+    __shared__ T tile[PRT][PNB + 1];
     __shared__ T prow[PNB];
     __shared__ T red_v[PTHREADS / 64];
     __shared__ int red_i[PTHREADS / 64];
     __shared__ int s_piv;
 
     const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
-    const int r0 = g * PR;
-    const int rows = min(PR, m - r0);
+    const int r0 = g * PRT;
+    const int rows = min(PRT, m - r0);
     const int lane = tid & 63, wave = tid >> 6;
 
-    {   // all loads in flight before the LDS stores (PR x PNB / PTHREADS per thread)
-        constexpr int PER = PR * PNB / PTHREADS;
+    {   // all loads in flight before the LDS stores (PRT x PNB / PTHREADS per thread)
+        constexpr int PER = PRT * PNB / PTHREADS;
         T v[PER];
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
@@ -241,14 +246,14 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
         {
 #pragma clang fp contract(off)
             const T piv = prow[j];
-            const int r = tid & (PR - 1), h = tid / PR;
+            const int r = tid & (PRT - 1), h = tid / PRT;
             const bool act = r < rows && r0 + r > j;
             T mult = T(0);
             if (act) mult = tile[r][j] / piv;
             __syncthreads();
             if (act) {
                 if (h == 0) tile[r][j] = mult;
-                for (int c = j + 1 + h; c < w; c += PTHREADS / PR) tile[r][c] = tile[r][c] - mult * prow[c];
+                for (int c = j + 1 + h; c < w; c += PTHREADS / PRT) tile[r][c] = tile[r][c] - mult * prow[c];
             }
             __syncthreads();
         }
@@ -408,7 +413,7 @@ __global__ __launch_bounds__(256) void k_trsm_llu(T *__restrict__ A, long lda, i
 template <class T>
 static int panel(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, int64_t *ipiv,
                  int64_t *info, size_t n, const LuWork &lw, unsigned *bar) {
-    const unsigned G = ceil_div(m, PR);
+    const unsigned G = ceil_div(m, PanelRows<T>::value);
     if (G > GMAX) {
         set_error("LU panel of %zu rows needs %u workgroups (max %d)", m, G, GMAX);
         return RLG_E_INVALID_ARG;
