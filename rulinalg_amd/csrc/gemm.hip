@@ -148,8 +148,19 @@ __global__ __launch_bounds__(GTHREADS) void k_gemm(
         __syncthreads();
     }
 
+    // epilogue in four row groups: the 16 loads of C of a group are issued
+    // before its stores (a load behind a store to C would wait for it)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i) {
+        T cin[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wm + i * 16 + Mfma<T>::row(lk, r);
+                const int gj = n0 + wn + j * 16 + lr;
+                cin[j][r] = (beta != T(0) && gi < M && gj < N) ? C[(long)gi * ldc + gj] : T(0);
+            }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -157,12 +168,12 @@ __global__ __launch_bounds__(GTHREADS) void k_gemm(
                 const int gi = m0 + wm + i * 16 + Mfma<T>::row(lk, r);
                 const int gj = n0 + wn + j * 16 + lr;
                 if (gi < M && gj < N) {
-                    T *p = C + (long)gi * ldc + gj;
                     T v = alpha * acc[i][j][r];
-                    if (beta != T(0)) v += beta * *p;
-                    *p = v;
+                    if (beta != T(0)) v += beta * cin[j][r];
+                    C[(long)gi * ldc + gj] = v;
                 }
             }
+    }
 }
 
 // ---------------------------------------------------------------- low precision
@@ -182,6 +193,36 @@ template <> struct LowpMfma<__bf16> {
     }
     static __device__ __forceinline__ __bf16 cvt(float x) { return (__bf16)x; }
 };
+
+// C = alpha * acc + beta * C for one wave's 64 x 64 sub-tile at (row0, col0),
+// FP32 accumulator layout.  Four row groups; the 16 loads of C of a group are
+// issued before its stores (a load behind a store to C would wait for it).
+__device__ __forceinline__ void epilogue_f32(const f4 (&acc)[4][4], float *__restrict__ C, long ldc,
+                                             int M, int N, int row0, int col0, int lr, int lk,
+                                             float alpha, float beta) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float cin[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = row0 + i * 16 + lk * 4 + r, gj = col0 + j * 16 + lr;
+                cin[j][r] = (beta != 0.f && gi < M && gj < N) ? C[(long)gi * ldc + gj] : 0.f;
+            }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = row0 + i * 16 + lk * 4 + r, gj = col0 + j * 16 + lr;
+                if (gi < M && gj < N) {
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.f) v += beta * cin[j][r];
+                    C[(long)gi * ldc + gj] = v;
+                }
+            }
+    }
+}
 
 constexpr int LBK = 64;          // k per stage (two MFMA k-steps of 32)
 constexpr int LPAD = 8;          // halves of padding per staged row
@@ -269,21 +310,7 @@ __global__ __launch_bounds__(GTHREADS) void k_gemm_lowp(
         __syncthreads();
     }
 
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gi = m0 + wm + i * 16 + lk * 4 + r;
-                const int gj = n0 + wn + j * 16 + lr;
-                if (gi < M && gj < N) {
-                    float *p = C + (long)gi * ldc + gj;
-                    float v = alpha * acc[i][j][r];
-                    if (beta != 0.f) v += beta * *p;
-                    *p = v;
-                }
-            }
+    epilogue_f32(acc, C, ldc, M, N, m0 + wm, n0 + wn, lr, lk, alpha, beta);
 }
 
 // ---------------------------------------------------------------- packed low precision
@@ -412,21 +439,7 @@ __global__ __launch_bounds__(GTHREADS) void k_gemm_nt_lowp(
         __syncthreads();
     }
 
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gi = m0 + wm + i * 16 + lk * 4 + r;
-                const int gj = n0 + wn + j * 16 + lr;
-                if (gi < M && gj < N) {
-                    float *p = C + (long)gi * ldc + gj;
-                    float v = alpha * acc[i][j][r];
-                    if (beta != 0.f) v += beta * *p;
-                    *p = v;
-                }
-            }
+    epilogue_f32(acc, C, ldc, M, N, m0 + wm, n0 + wn, lr, lk, alpha, beta);
 }
 
 size_t gemm_lowp_pack_bytes(size_t m, size_t n, size_t k) {
