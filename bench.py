@@ -240,6 +240,16 @@ def main():
                 ("flops_per_launch" if bound == "mfma" else "bytes_per_launch"): wk / cnt}
 
     roof = roof_of(name, cnt, ms, wk)
+    # HBM bytes per launch of the dominant class, from the FETCH_SIZE /
+    # WRITE_SIZE counter runs recorded for this order (scaled by a plain copy
+    # of known size); null when no record exists for this order
+    tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_n{n}.json")
+    if os.path.exists(tpath):
+        rec = json.load(open(tpath))
+        t = rec.get("classes", {}).get(name)
+        if t:
+            roof["traffic"] = t["fetch_bytes_per_launch"] + t["write_bytes_per_launch"]
+            roof["traffic_source"] = os.path.relpath(tpath, ROOT)
     breakdown = {k: {"launches": v[0] // prof_steps, "ms_per_step": round(v[1] / prof_steps, 3),
                      "roofline_frac": roof_of(k, *v)["frac"]}
                  for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}

@@ -74,6 +74,16 @@ size_t lu_lowp_work_bytes(size_t n);
 int lu_factor_lowp(hipStream_t s, int prec, size_t n, float *a, size_t lda,
                    int64_t *ipiv, int64_t *info, void *work);
 
+// Tournament-pivoting FP32 panel (tslu.hip), used by lu_factor_lowp: the
+// workspace of tslu_work_bytes() is cleared by tslu_reset once per
+// factorization; `epoch` increases by one per panel.  Panels of up to
+// tslu_max_rows() rows.
+size_t tslu_work_bytes();
+size_t tslu_max_rows();
+int tslu_reset(hipStream_t s, void *work);
+int tslu_panel(hipStream_t s, float *a, size_t lda, size_t k, size_t m, size_t w, int64_t *ipiv,
+               int64_t *info, size_t n, void *work, unsigned epoch, int *timeout);
+
 // Device flag raised by a panel barrier that gave up waiting (see lu.hip);
 // nonzero after lu_factor means the factorization is invalid.
 int *lu_timeout_flag(void *work, size_t n, size_t elem);

@@ -19,6 +19,7 @@
 // subtract (lu.rs:603-616), so a matrix no wider than one panel is factored
 // bit-identically to the reference.
 #include <climits>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -410,6 +411,26 @@ __global__ __launch_bounds__(256) void k_trsm_llu(T *__restrict__ A, long lda, i
 
 // ---------------------------------------------------------------- driver
 
+// Set by lu_factor_lowp for its duration: FP32 panels are then factored with
+// tournament pivoting (tslu.hip) in this workspace, one epoch per panel.
+static thread_local void *g_tslu_work = nullptr;
+static thread_local unsigned g_tslu_epoch = 0;
+
+template <class T>
+static int panel(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, int64_t *ipiv,
+                 int64_t *info, size_t n, const LuWork &lw, unsigned *bar);
+
+template <class T>
+static int panel_any(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, int64_t *ipiv,
+                     int64_t *info, size_t n, const LuWork &lw, unsigned *bar) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (g_tslu_work && m <= tslu_max_rows())
+            return tslu_panel(s, a, lda, k, m, w, ipiv, info, n, g_tslu_work, ++g_tslu_epoch,
+                              lw.timeout);
+    }
+    return panel<T>(s, a, lda, k, m, w, ipiv, info, n, lw, bar);
+}
+
 template <class T>
 static int panel(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, int64_t *ipiv,
                  int64_t *info, size_t n, const LuWork &lw, unsigned *bar) {
@@ -491,7 +512,7 @@ static int outer_panel(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv,
                        const LuWork &lw, size_t k0, size_t kb) {
     for (size_t j0 = k0; j0 < k0 + kb; j0 += PNB) {
         const size_t jb = k0 + kb - j0 < (size_t)PNB ? k0 + kb - j0 : PNB;
-        RLG_TRY(panel<T>(s, a, lda, j0, n - j0, jb, ipiv, info, n, lw, lw.bar + j0 / PNB));
+        RLG_TRY(panel_any<T>(s, a, lda, j0, n - j0, jb, ipiv, info, n, lw, lw.bar + j0 / PNB));
         // inner swaps on the other columns of the outer block
         RLG_TRY(laswp<T>(s, a, lda, ipiv, j0, jb, k0, j0, j0 + jb, k0 + kb, lw));
         if (j0 + jb < k0 + kb) {
@@ -579,17 +600,25 @@ template int lu_factor<float>(hipStream_t, size_t, float *, size_t, int64_t *, i
 
 // The trailing updates are at most n x n with depth ONB.
 size_t lu_lowp_work_bytes(size_t n) {
-    return align_up(lu_work_bytes(n, 4)) + gemm_lowp_pack_bytes(n, n, ONB);
+    return align_up(lu_work_bytes(n, 4)) + align_up(gemm_lowp_pack_bytes(n, n, ONB)) +
+           tslu_work_bytes();
 }
 
 int lu_factor_lowp(hipStream_t s, int prec, size_t n, float *a, size_t lda, int64_t *ipiv,
                    int64_t *info, void *work) {
+    char *w = static_cast<char *>(work);
     g_lowp_prec = prec;
-    g_lowp_pack = static_cast<char *>(work) + align_up(lu_work_bytes(n, 4));
+    g_lowp_pack = w + align_up(lu_work_bytes(n, 4));
     g_lowp_pack_bytes = gemm_lowp_pack_bytes(n, n, ONB);
-    const int st = lu_factor_impl<float>(s, n, a, lda, ipiv, info, work, &update_lowp);
+    g_tslu_work = w + align_up(lu_work_bytes(n, 4)) + align_up(g_lowp_pack_bytes);
+    g_tslu_epoch = 0;
+    int st = tslu_reset(s, g_tslu_work);
+    if (st == RLG_OK)
+        st = lu_factor_impl<float>(s, n, a, lda, ipiv, info, work,
+                                   prec == RLG_PREC_F32 ? &update_native<float> : &update_lowp);
     g_lowp_pack = nullptr;
     g_lowp_pack_bytes = 0;
+    g_tslu_work = nullptr;
     return st;
 }
 

@@ -292,7 +292,7 @@ int solve_mixed_dev(hipStream_t s, size_t n, const double *a, size_t lda, size_t
         RLG_TRY(lu32.alloc(n * n * 4, s));
         RLG_TRY(ipiv.alloc(n * 8, s));
         RLG_TRY(info.alloc(8, s));
-        RLG_TRY(work.alloc(o.factor_prec == RLG_PREC_F32 ? lu_work_bytes(n, 4) : lu_lowp_work_bytes(n), s));
+        RLG_TRY(work.alloc(lu_lowp_work_bytes(n), s));
         RLG_TRY(t32.alloc(nrhs * n * 4, s));
         RLG_TRY(tmp32.alloc(nrhs * n * 4, s));
         int64_t info_h = 0;
@@ -317,12 +317,10 @@ int solve_mixed_dev(hipStream_t s, size_t n, const double *a, size_t lda, size_t
                                    lu32.as<float>(), (long)n);
                 RLG_LAUNCHED();
             }
-            if (o.factor_prec == RLG_PREC_F32)
-                RLG_TRY(lu_factor<float>(s, n, lu32.as<float>(), n, ipiv.as<int64_t>(),
-                                         info.as<int64_t>(), work.p));
-            else
-                RLG_TRY(lu_factor_lowp(s, o.factor_prec, n, lu32.as<float>(), n, ipiv.as<int64_t>(),
-                                       info.as<int64_t>(), work.p));
+            // FP32 panels with tournament pivoting; trailing updates in FP32,
+            // FP16 or BF16 (FP32 accumulation)
+            RLG_TRY(lu_factor_lowp(s, o.factor_prec, n, lu32.as<float>(), n, ipiv.as<int64_t>(),
+                                   info.as<int64_t>(), work.p));
             RLG_HIP(hipMemcpyAsync(&info_h, info.p, 8, hipMemcpyDeviceToHost, s));
             RLG_HIP(hipMemcpyAsync(&timeout, lu_timeout_flag(work.p, n, 4), 4, hipMemcpyDeviceToHost, s));
             rep.factor_ms = tf.stop();
