@@ -278,11 +278,105 @@ def main():
         del work
         torch.cuda.empty_cache()
         result.update(extras(lib, torch, dev, sptr))
+        result["batched"] = batched(lib, torch, dev)
 
     # ------------------------------------------------------------ CPU baseline
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_n, n)
     print(json.dumps(result), flush=True)
+
+
+def batched(lib, torch, dev, count=20000, nmin=8, nmax=512, seed=30):
+    """20,000 independent systems, n uniform on [8, 512], half LU (Gaussian)
+    and half Cholesky (G G^T / n + I), 1-8 right-hand sides each, solved in
+    one persistent launch, plain FP64 and FP32-factor + FP64 refinement.
+    Inputs are generated on the GPU and restored from a pristine copy before
+    each run (outside the timed region)."""
+    from rulinalg_amd import _lib
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, count)
+    rs = rng.integers(1, 9, count)
+    kinds = np.arange(count) % 2  # 0: LU, 1: Cholesky
+    a_off = np.concatenate([[0], np.cumsum(ns.astype(np.int64) ** 2)])
+    b_off = np.concatenate([[0], np.cumsum(ns.astype(np.int64) * rs)])
+    p_off = np.concatenate([[0], np.cumsum(ns.astype(np.int64))])
+    A0 = torch.empty(int(a_off[-1]), dtype=torch.float64, device=dev)
+    B0 = torch.empty(int(b_off[-1]), dtype=torch.float64, device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    t0 = time.time()
+    for n in np.unique(ns):
+        for kind in (0, 1):
+            idx = np.nonzero((ns == n) & (kinds == kind))[0]
+            if len(idx) == 0:
+                continue
+            m = torch.randn(len(idx), n, n, dtype=torch.float64, device=dev, generator=g)
+            if kind == 1:
+                mt = m.transpose(1, 2).contiguous()
+                m = torch.bmm(m, mt) / n + torch.eye(n, dtype=torch.float64, device=dev)
+            for q, i in enumerate(idx):
+                A0[int(a_off[i]):int(a_off[i + 1])].copy_(m[q].reshape(-1))
+    B0.normal_(generator=g)
+    A = torch.empty_like(A0)
+    B = torch.empty_like(B0)
+    P = torch.empty(int(p_off[-1]), dtype=torch.int64, device=dev)
+    items = (_lib.BatchItem * count)()
+    for i in range(count):
+        items[i] = _lib.BatchItem(int(kinds[i]), int(ns[i]), int(rs[i]), -1,
+                                  A.data_ptr() + 8 * int(a_off[i]), B.data_ptr() + 8 * int(b_off[i]),
+                                  P.data_ptr() + 8 * int(p_off[i]))
+    it_dev = torch.empty(C.sizeof(items), dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    assert lib.rlg_dev_upload(it_dev.data_ptr(), C.addressof(items), C.sizeof(items)) == 0
+    log(f"batched: {count} systems generated in {time.time() - t0:.1f}s")
+    work = float(np.sum(np.where(kinds == 0, 2.0 / 3.0, 1.0 / 3.0) * ns.astype(float) ** 3 +
+                        2.0 * ns.astype(float) ** 2 * rs))
+
+    def run(mixed):
+        A.copy_(A0)
+        B.copy_(B0)
+        torch.cuda.synchronize()
+        fb = C.c_size_t(0)
+        t0 = time.perf_counter()
+        if mixed:
+            st = lib.rlg_batch_solve_mixed_dev(count, it_dev.data_ptr(), 30, 1e-13, C.byref(fb), None)
+        else:
+            st = lib.rlg_batch_solve_f64_dev(count, it_dev.data_ptr(), None)
+        assert st == 0, _lib.last_error()
+        assert lib.rlg_synchronize() == 0
+        return (time.perf_counter() - t0) * 1e3, fb.value
+
+    out = {"systems": count, "n_range": [nmin, nmax], "rhs_range": [1, 8],
+           "flop_per_batch": work}
+    for mixed in (False, True):
+        run(mixed)  # warm (and the autotuner's one-time sweep, if any)
+        times = []
+        for _ in range(3):
+            ms, fb = run(mixed)
+            times.append(ms)
+        # statuses and backward errors of a sample (host check)
+        it_back = (_lib.BatchItem * count)()
+        assert lib.rlg_dev_download(C.addressof(it_back), it_dev.data_ptr(), C.sizeof(it_back)) == 0
+        bad = sum(1 for i in range(count) if it_back[i].status != 0)
+        etas = []
+        for i in rng.choice(count, 24, replace=False):
+            n, r = int(ns[i]), int(rs[i])
+            a = A0[int(a_off[i]):int(a_off[i + 1])].view(n, n).cpu().numpy()
+            b = B0[int(b_off[i]):int(b_off[i + 1])].view(r, n).cpu().numpy()
+            x = B[int(b_off[i]):int(b_off[i + 1])].view(r, n).cpu().numpy()
+            for q in range(r):
+                etas.append(np.abs(b[q] - a @ x[q]).max() /
+                            (np.abs(a).sum(1).max() * np.abs(x[q]).max() + np.abs(b[q]).max()))
+        best = min(times)
+        key = "batched_mixed" if mixed else "batched_plain"
+        out[key] = {"ms": round(best, 3), "solves_per_s": round(count / (best / 1e3), 1),
+                    "gflops": round(work / (best / 1e3) / 1e9, 1),
+                    "frac_fp64_peak": round(work / (best / 1e3) / 78.6e12, 4),
+                    "failed_items": bad, "fallbacks": fb, "backward_error_max_sample": max(etas)}
+        log(f"{key}: {best:.1f} ms, {count / (best / 1e3):.0f} solves/s, eta max {max(etas):.1e}")
+    del A, B, A0, B0, P
+    torch.cuda.empty_cache()
+    return out
 
 
 def extras(lib, torch, dev, sptr):
