@@ -69,7 +69,7 @@ __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, i
 }
 
 template <class T, bool TRANSB, bool LOWER>
-__global__ __launch_bounds__(GTHREADS) void k_gemm(
+__global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
     int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
     const T *__restrict__ B, long ldb, T beta, T *__restrict__ C, long ldc,
     int tiles_m, int tiles_n) {
