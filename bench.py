@@ -112,14 +112,18 @@ def main():
 
     s = torch.logspace(0.0, float(np.log10(args.kappa)), n, dtype=torch.float64, device=dev)
     A = torch.empty(n, n, dtype=torch.float64, device=dev)
+    uq = vt = None
     if args.gen == "qr":
-        us = orthogonal(20) * s.unsqueeze(0)
+        uq = orthogonal(20)
+        us = uq * s.unsqueeze(0)
         vt = orthogonal(21).t().contiguous()
         torch.cuda.synchronize()
         assert lib.rlg_gemm_f64_dev(n, n, n, 1.0, us.data_ptr(), n, vt.data_ptr(), n, 0.0,
                                     A.data_ptr(), n, sptr) == 0, _lib.last_error()
         assert lib.rlg_synchronize() == 0
-        del us, vt
+        del us
+        if args.no_extras:
+            uq = vt = None
     else:
         # U and V as products of 8 Householder reflectors each (exactly
         # orthogonal, same singular values; no LAPACK call involved)
@@ -275,7 +279,66 @@ def main():
 
     # ------------------------------------------------------------ extras
     if not args.no_extras:
-        del work
+        variants = {}
+
+        def mixed_once(a, b, x, nr, prec):
+            o = _lib.MixedOpts(PREC[prec], 0, 1e-13, 30, 1)
+            r = _lib.MixedReport()
+            assert lib.rlg_synchronize() == 0
+            t0 = time.perf_counter()
+            st = lib.rlg_solve_mixed_f64_dev(n, a.data_ptr(), n, nr, b.data_ptr(), n, x.data_ptr(), n,
+                                             C.byref(o), C.byref(r), sptr)
+            assert st == 0, _lib.last_error()
+            assert lib.rlg_synchronize() == 0
+            ms = (time.perf_counter() - t0) * 1e3
+            st = lib.rlg_backward_error_f64_dev(n, a.data_ptr(), n, nr, b.data_ptr(), n, x.data_ptr(),
+                                                n, C.byref(eta), sptr)
+            assert st == 0, _lib.last_error()
+            return {"ms": round(ms, 3), "backward_error": eta.value, "ir_iterations": r.iterations,
+                    "fell_back": r.fell_back, "factor_prec": prec}
+
+        # 32 right-hand sides at kappa 1e4, both factor precisions
+        g.manual_seed(23)
+        B32 = torch.randn(32, n, dtype=torch.float64, device=dev, generator=g)
+        X32 = torch.empty_like(B32)
+        torch.cuda.synchronize()
+        for prec in ("f32", args.prec):
+            mixed_once(A, B32, X32, 32, prec)
+            variants[f"mixed_{prec}_32rhs_kappa1e4"] = mixed_once(A, B32, X32, 32, prec)
+        # plain FP64 LU + 16 right-hand sides (factor and solve)
+        B16, X16 = B32[:16].contiguous(), X32[:16].contiguous()
+        assert lib.rlg_dev_copy(work.data_ptr(), A.data_ptr(), A.numel() * 8, sptr) == 0
+        assert lib.rlg_synchronize() == 0
+        t0 = time.perf_counter()
+        h = C.c_void_p()
+        assert lib.rlg_lu_decompose_f64_dev(n, work.data_ptr(), n, sptr, C.byref(h)) == 0
+        assert lib.rlg_lu_solve_f64_dev(h, 16, B16.data_ptr(), n, X16.data_ptr(), n, sptr) == 0
+        assert lib.rlg_synchronize() == 0
+        ms = (time.perf_counter() - t0) * 1e3
+        lib.rlg_lu_free(h)
+        assert lib.rlg_backward_error_f64_dev(n, A.data_ptr(), n, 16, B16.data_ptr(), n,
+                                              X16.data_ptr(), n, C.byref(eta), sptr) == 0
+        fl = 2.0 / 3.0 * n ** 3 + 2.0 * n * n * 16
+        variants["plain_lu_16rhs"] = {"ms": round(ms, 3), "tflops": round(fl / (ms / 1e3) / 1e12, 2),
+                                      "frac_fp64_peak": round(fl / (ms / 1e3) / 78.6e12, 4),
+                                      "backward_error": eta.value}
+        # kappa 1e8: refinement from the low-precision factors stalls and the
+        # FP64 factorization takes over
+        if uq is not None:
+            s8 = torch.logspace(0.0, 8.0, n, dtype=torch.float64, device=dev)
+            us8 = uq * s8.unsqueeze(0)
+            A8 = torch.empty_like(A)
+            torch.cuda.synchronize()
+            assert lib.rlg_gemm_f64_dev(n, n, n, 1.0, us8.data_ptr(), n, vt.data_ptr(), n, 0.0,
+                                        A8.data_ptr(), n, sptr) == 0
+            assert lib.rlg_synchronize() == 0
+            del us8
+            variants[f"mixed_{args.prec}_1rhs_kappa1e8"] = mixed_once(A8, B, X, nrhs, args.prec)
+            del A8
+        for k_, v_ in variants.items():
+            log(f"{k_}: {v_}")
+        result["variants"] = variants
+        del work, uq, vt, B32, X32
         torch.cuda.empty_cache()
         result.update(extras(lib, torch, dev, sptr))
         result["batched"] = batched(lib, torch, dev)
