@@ -192,8 +192,9 @@ __global__ __launch_bounds__(64) void k_tslu(float *__restrict__ A, long lda, in
     __builtin_amdgcn_wave_barrier();
     // publish the winners' original rows
     for (int j = 0; j < TW; ++j) {
-        const int h = who[j], lr = (h >> 1) + 64 * (h & 1);
+        // entries j >= w of order/who are never written: use row 0 for them
         const bool ok = j < w && order[j] >= 0;
+        const int h = ok ? who[j] : 0, lr = (h >> 1) + 64 * (h & 1);
         tw.rows[((long)g * TW + j) * TW + lane] = ok ? orig[lr][lane] : 0.f;
         if (lane == 0) tw.ids[g * TW + j] = ok ? order[j] : -1;
     }
@@ -219,8 +220,8 @@ __global__ __launch_bounds__(64) void k_tslu(float *__restrict__ A, long lda, in
             tslu_gepp(a0, a1, cur_id[lane], cur_id[64 + lane], w, order, who, zstep);
             __builtin_amdgcn_wave_barrier();
             for (int j = 0; j < TW; ++j) {
-                const int h = who[j], lr = (h >> 1) + 64 * (h & 1);
                 const bool ok = j < w && order[j] >= 0;
+                const int h = ok ? who[j] : 0, lr = (h >> 1) + 64 * (h & 1);
                 tw.rows[((long)g * TW + j) * TW + lane] = ok ? cur[lr][lane] : 0.f;
                 if (lane == 0) tw.ids[g * TW + j] = ok ? order[j] : -1;
             }
