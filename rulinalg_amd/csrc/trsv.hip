@@ -36,6 +36,24 @@ constexpr int CPW = 64;     // entries updated per workgroup (transposed form)
 
 enum TriMode { LOWER_UNIT = 0, LOWER = 1, UPPER = 2, LOWER_T = 3 };
 
+// x of lane `src` to every lane; `src` is wave-uniform, so this is a
+// v_readlane (scalar result, a few cycles) rather than an LDS permute.
+__device__ __forceinline__ float lane_val(float x, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
+}
+__device__ __forceinline__ double lane_val(double x, int src) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// x / d on the solve's serial chain: FP64 divides (the reference divides);
+// FP32 (the refinement's correction solves) multiplies by a reciprocal that
+// every lane computed up front.
+__device__ __forceinline__ double diag_div(double x, double d, double) { return x / d; }
+__device__ __forceinline__ float diag_div(float x, float, float rd) { return x * rd; }
+
 template <int MODE> __device__ __forceinline__ constexpr bool forward() {
     return MODE == LOWER_UNIT || MODE == LOWER;
 }
@@ -101,22 +119,24 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
                 const T xp = fwd ? xq0 : xq1;
                 const int ps = fwd ? bs0 : bs1;
                 for (int c = 0; c < ps; ++c) {
-                    const T xc = __shfl(xp, c, 64);
+                    const T xc = lane_val(xp, c);
                     const T m = MODE == LOWER_T ? tl[2][c][lane] : tl[2][lane][c];
                     x = x - m * xc;
                 }
             }
             const T(*d)[SB + 1] = tl[q];
+            const T dl = lane < bs ? d[lane][lane] : T(1);
+            const T rd = T(1) / dl;  // used by the FP32 form only
             if (fwd) {
                 for (int c = 0; c < bs; ++c) {
-                    if (MODE == LOWER && lane == c) x = x / d[c][c];
-                    const T xc = __shfl(x, c, 64);
+                    if (MODE == LOWER && lane == c) x = diag_div(x, dl, rd);
+                    const T xc = lane_val(x, c);
                     if (lane > c) x = x - d[lane][c] * xc;
                 }
             } else {
                 for (int c = bs - 1; c >= 0; --c) {
-                    if (lane == c) x = x / d[c][c];
-                    const T xc = __shfl(x, c, 64);
+                    if (lane == c) x = diag_div(x, dl, rd);
+                    const T xc = lane_val(x, c);
                     if (lane < c) x = x - (MODE == UPPER ? d[lane][c] : d[c][lane]) * xc;
                 }
             }
