@@ -205,12 +205,17 @@ enum {
     RLG_REFINE_IR = 0,     /* classical iterative refinement */
     RLG_REFINE_GMRES = 1   /* GMRES-IR: GMRES preconditioned by the factors */
 };
+enum {
+    RLG_PIVOT_PARTIAL = 0,     /* partial pivoting, as PartialPivLu (lu.rs:170-178) */
+    RLG_PIVOT_TOURNAMENT = 1   /* tournament pivoting (TSLU) on the FP32 panels */
+};
 typedef struct rlg_mixed_opts {
     int factor_prec;      /* RLG_PREC_* */
     int refine;           /* RLG_REFINE_* */
     double tol;           /* target normwise backward error (1e-13) */
     int max_iter;         /* refinement steps before declaring a stall */
     int fallback;         /* nonzero: fall back to plain FP64 on a stall */
+    int pivot;            /* RLG_PIVOT_* for the low-precision factorization */
 } rlg_mixed_opts;
 typedef struct rlg_mixed_report {
     int iterations;        /* refinement steps taken (outer steps) */

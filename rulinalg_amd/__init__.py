@@ -460,13 +460,13 @@ class MixedOptions:
     """Options of the mixed-precision refined solve (include/rlgpu.h)."""
 
     def __init__(self, factor_prec=_lib.PREC_F32, refine=_lib.REFINE_IR, tol=1e-13, max_iter=30,
-                 fallback=True):
+                 fallback=True, pivot=_lib.PIVOT_PARTIAL):
         self.factor_prec, self.refine, self.tol = factor_prec, refine, tol
-        self.max_iter, self.fallback = max_iter, fallback
+        self.max_iter, self.fallback, self.pivot = max_iter, fallback, pivot
 
     def to_c(self):
         return _lib.MixedOpts(self.factor_prec, self.refine, self.tol, self.max_iter,
-                              1 if self.fallback else 0)
+                              1 if self.fallback else 0, self.pivot)
 
 
 def solve_mixed(a, b, options=None):

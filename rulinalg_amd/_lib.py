@@ -19,6 +19,7 @@ STATUS_NAMES = {
 
 PREC_F64, PREC_F32, PREC_F16, PREC_BF16 = 0, 1, 2, 3
 REFINE_IR, REFINE_GMRES = 0, 1
+PIVOT_PARTIAL, PIVOT_TOURNAMENT = 0, 1
 KIND_LU, KIND_CHOL = 0, 1
 
 sz = C.c_size_t
@@ -35,7 +36,7 @@ PI64 = C.POINTER(C.c_int64)
 
 class MixedOpts(C.Structure):
     _fields_ = [("factor_prec", i32), ("refine", i32), ("tol", dbl), ("max_iter", i32),
-                ("fallback", i32)]
+                ("fallback", i32), ("pivot", i32)]
 
 
 class MixedReport(C.Structure):

@@ -71,7 +71,10 @@ int lu_factor(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv,
 // Its workspace is lu_lowp_work_bytes(n) bytes (lu_work_bytes plus room for
 // the packed low-precision operands of the trailing update).
 size_t lu_lowp_work_bytes(size_t n);
-int lu_factor_lowp(hipStream_t s, int prec, size_t n, float *a, size_t lda,
+// `pivot`: RLG_PIVOT_PARTIAL (the FP32 form of k_lu_panel) or
+// RLG_PIVOT_TOURNAMENT (tslu.hip); `prec` RLG_PREC_F32 keeps the trailing
+// updates in FP32.
+int lu_factor_lowp(hipStream_t s, int prec, int pivot, size_t n, float *a, size_t lda,
                    int64_t *ipiv, int64_t *info, void *work);
 
 // Tournament-pivoting FP32 panel (tslu.hip), used by lu_factor_lowp: the

@@ -26,11 +26,11 @@
 namespace rlg {
 
 constexpr int PNB = 64;    // inner panel width
-constexpr int PR = 128;    // panel rows per workgroup (LDS-resident), FP64
-// Panel rows per workgroup by element type: FP32 panels take 256 rows (the
-// same 66.5 KB of LDS), halving the workgroups that meet at every column.
+// Panel rows per workgroup (LDS-resident): 256 (133 KB of LDS in FP64, one
+// workgroup per CU).  A grid barrier costs about 85 ns per participating
+// workgroup on MI355X, so fewer, taller workgroups make every column cheaper.
+constexpr int PR = 256;
 template <class T> struct PanelRows { static constexpr int value = PR; };
-template <> struct PanelRows<float> { static constexpr int value = 2 * PR; };
 constexpr int ONB = 256;   // outer block: depth of the trailing GEMM
 constexpr int PTHREADS = 256;
 
@@ -604,15 +604,19 @@ size_t lu_lowp_work_bytes(size_t n) {
            tslu_work_bytes();
 }
 
-int lu_factor_lowp(hipStream_t s, int prec, size_t n, float *a, size_t lda, int64_t *ipiv,
+int lu_factor_lowp(hipStream_t s, int prec, int pivot, size_t n, float *a, size_t lda, int64_t *ipiv,
                    int64_t *info, void *work) {
     char *w = static_cast<char *>(work);
     g_lowp_prec = prec;
     g_lowp_pack = w + align_up(lu_work_bytes(n, 4));
     g_lowp_pack_bytes = gemm_lowp_pack_bytes(n, n, ONB);
-    g_tslu_work = w + align_up(lu_work_bytes(n, 4)) + align_up(g_lowp_pack_bytes);
+    g_tslu_work = nullptr;
     g_tslu_epoch = 0;
-    int st = tslu_reset(s, g_tslu_work);
+    int st = RLG_OK;
+    if (pivot == RLG_PIVOT_TOURNAMENT) {
+        g_tslu_work = w + align_up(lu_work_bytes(n, 4)) + align_up(g_lowp_pack_bytes);
+        st = tslu_reset(s, g_tslu_work);
+    }
     if (st == RLG_OK)
         st = lu_factor_impl<float>(s, n, a, lda, ipiv, info, work,
                                    prec == RLG_PREC_F32 ? &update_native<float> : &update_lowp);

@@ -317,10 +317,10 @@ int solve_mixed_dev(hipStream_t s, size_t n, const double *a, size_t lda, size_t
                                    lu32.as<float>(), (long)n);
                 RLG_LAUNCHED();
             }
-            // FP32 panels with tournament pivoting; trailing updates in FP32,
-            // FP16 or BF16 (FP32 accumulation)
-            RLG_TRY(lu_factor_lowp(s, o.factor_prec, n, lu32.as<float>(), n, ipiv.as<int64_t>(),
-                                   info.as<int64_t>(), work.p));
+            // FP32 panels (partial or tournament pivoting); trailing updates in
+            // FP32, FP16 or BF16 (FP32 accumulation)
+            RLG_TRY(lu_factor_lowp(s, o.factor_prec, o.pivot, n, lu32.as<float>(), n,
+                                   ipiv.as<int64_t>(), info.as<int64_t>(), work.p));
             RLG_HIP(hipMemcpyAsync(&info_h, info.p, 8, hipMemcpyDeviceToHost, s));
             RLG_HIP(hipMemcpyAsync(&timeout, lu_timeout_flag(work.p, n, 4), 4, hipMemcpyDeviceToHost, s));
             rep.factor_ms = tf.stop();
@@ -430,6 +430,7 @@ void rlg_mixed_opts_default(rlg_mixed_opts *o) {
     o->tol = 1e-13;
     o->max_iter = 30;
     o->fallback = 1;
+    o->pivot = RLG_PIVOT_PARTIAL;
 }
 
 int rlg_solve_mixed_f64_dev(size_t n, const double *a, size_t lda, size_t nrhs, const double *b,
@@ -440,6 +441,10 @@ int rlg_solve_mixed_f64_dev(size_t n, const double *a, size_t lda, size_t nrhs, 
     if (opts) o = *opts; else rlg_mixed_opts_default(&o);
     if (o.factor_prec < RLG_PREC_F64 || o.factor_prec > RLG_PREC_BF16) {
         set_error("unknown factor precision %d", o.factor_prec);
+        return RLG_E_INVALID_ARG;
+    }
+    if (o.pivot != RLG_PIVOT_PARTIAL && o.pivot != RLG_PIVOT_TOURNAMENT) {
+        set_error("unknown pivoting %d", o.pivot);
         return RLG_E_INVALID_ARG;
     }
     Device *d;

@@ -415,6 +415,23 @@ def test_mixed_refined_well_conditioned(prec):
         assert rep["fell_back"] == 0
 
 
+@pytest.mark.parametrize("prec", [_lib.PREC_F32, _lib.PREC_F16])
+@pytest.mark.parametrize("n", [200, 1500])
+def test_mixed_tournament_pivoting(prec, n):
+    """TSLU panels (n = 1500: twelve 128-row chunks, a four-level tree):
+    refinement must converge without the FP64 fallback, to the same bar as
+    partial pivoting."""
+    a = wl.conditioned(n, 1e3, 5, 6)
+    b = wl.gaussian(2, n, 7)
+    x, rep = rl.solve_mixed(a, b, rl.MixedOptions(factor_prec=prec, pivot=_lib.PIVOT_TOURNAMENT))
+    assert rep["converged"] == 1 and rep["fell_back"] == 0
+    for i in range(2):
+        assert eta(a, x[i], b[i]) <= 1e-13
+    with pytest.raises(rl.Error) as e:  # an unknown pivoting mode is an argument error
+        rl.solve_mixed(a, b, rl.MixedOptions(pivot=7))
+    assert e.value.kind() == rl.ErrorKind.InvalidArg
+
+
 def test_mixed_kappa_1e4_f32():
     n = 512
     a = wl.conditioned(n, 1e4, 4, 5)
