@@ -269,54 +269,95 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
 // ---------------------------------------------------------------- row swaps
 
 // Net effect of the swaps ipiv[k0 .. k0+cnt) as a list of (dst, src) rows:
-// after the swaps, row dst holds what row src held before.  One wave; the
-// rows outside [k0, k0+cnt) that get touched are tracked in a small table
-// searched with a ballot.
+// after the swaps, row dst holds what row src held before.  One wave walks
+// the swaps in order with every table in registers (entry e of a table at
+// lane e % 64, slot e / 64, read with readlane): `cont` holds what position
+// k0 + e now contains; rows outside [k0, k0+cnt) that get touched are kept
+// in (erow, esrc) and found with a ballot.  The list is then written with a
+// ballot prefix count.
+constexpr int SMS = ONB / 64;  // table slots per lane
+
+__device__ __forceinline__ int tab_get(const int (&t)[SMS], int e) {
+    int v = t[0];
+#pragma unroll
+    for (int i = 1; i < SMS; ++i)
+        if ((e >> 6) == i) v = t[i];
+    return __builtin_amdgcn_readlane(v, e & 63);
+}
+__device__ __forceinline__ void tab_set(int (&t)[SMS], int e, int v) {
+    if ((int)threadIdx.x == (e & 63)) {
+#pragma unroll
+        for (int i = 0; i < SMS; ++i)
+            if ((e >> 6) == i) t[i] = v;
+    }
+}
+
 __global__ __launch_bounds__(64) void k_swap_map(const int64_t *__restrict__ ipiv, int k0, int cnt,
                                                  int *__restrict__ map) {
-    __shared__ int content[ONB];          // content[r - k0] for r in [k0, k0+cnt)
-    __shared__ int ext_row[ONB], ext_src[ONB];
-    __shared__ int piv[ONB];              // the step pivots, loaded once
-    __shared__ int n_ext;
     const int lane = threadIdx.x;
-    for (int i = lane; i < cnt; i += 64) {
-        content[i] = k0 + i;
-        piv[i] = (int)ipiv[k0 + i];
+    int cont[SMS], pv[SMS], erow[SMS], esrc[SMS];
+#pragma unroll
+    for (int i = 0; i < SMS; ++i) {
+        const int e = i * 64 + lane;
+        cont[i] = k0 + e;
+        pv[i] = e < cnt ? (int)ipiv[k0 + e] : k0 + e;
+        erow[i] = -1;
+        esrc[i] = -1;
     }
-    if (lane == 0) n_ext = 0;
-    __syncthreads();
+    int n_ext = 0;  // uniform
     for (int s = 0; s < cnt; ++s) {
-        const int r = k0 + s;
-        const int p = piv[s];
-        if (p == r) continue;
+        const int p = tab_get(pv, s);
+        if (p == k0 + s) continue;
         if (p < k0 + cnt) {
-            if (lane == 0) { int t = content[s]; content[s] = content[p - k0]; content[p - k0] = t; }
+            const int a = tab_get(cont, s), b = tab_get(cont, p - k0);
+            tab_set(cont, s, b);
+            tab_set(cont, p - k0, a);
         } else {
-            // find p among the external rows
-            int found = -1;
-            const int ne = n_ext;
-            for (int base = 0; base < ne; base += 64) {
-                const int q = base + lane;
-                const bool hit = q < ne && ext_row[q] == p;
-                const unsigned long long bal = __ballot(hit);
-                if (bal) { found = base + __ffsll((long long)bal) - 1; break; }
+            int idx = -1;
+#pragma unroll
+            for (int i = 0; i < SMS; ++i) {
+                const unsigned long long bal = __ballot(erow[i] == p);
+                if (idx < 0 && bal) idx = i * 64 + __ffsll((long long)bal) - 1;
             }
-            if (lane == 0) {
-                int idx = found;
-                if (idx < 0) { idx = ne; ext_row[idx] = p; ext_src[idx] = p; n_ext = ne + 1; }
-                int t = content[s]; content[s] = ext_src[idx]; ext_src[idx] = t;
+            if (idx < 0) {
+                idx = n_ext++;
+                tab_set(erow, idx, p);
+                tab_set(esrc, idx, p);
             }
+            const int a = tab_get(cont, s), e = tab_get(esrc, idx);
+            tab_set(cont, s, e);
+            tab_set(esrc, idx, a);
         }
-        __syncthreads();
     }
-    if (lane == 0) {
-        int c = 0;
-        for (int i = 0; i < cnt; ++i)
-            if (content[i] != k0 + i) { map[1 + c] = k0 + i; map[1 + 2 * ONB + c] = content[i]; ++c; }
-        for (int i = 0; i < n_ext; ++i)
-            if (ext_src[i] != ext_row[i]) { map[1 + c] = ext_row[i]; map[1 + 2 * ONB + c] = ext_src[i]; ++c; }
-        map[0] = c;
+    // the list: positions whose content changed, then touched outside rows
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < SMS; ++i) {
+        const int e = i * 64 + lane;
+        const bool mv = e < cnt && cont[i] != k0 + e;
+        const unsigned long long bal = __ballot(mv);
+        if (mv) {
+            const int at = c + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
+            map[1 + at] = k0 + e;
+            map[1 + 2 * ONB + at] = cont[i];
+        }
+        c += __popcll(bal);
     }
+#pragma unroll
+    for (int i = 0; i < SMS; ++i) {
+        const int e = i * 64 + lane;
+        const bool mv = e < n_ext && esrc[i] != erow[i];
+        const unsigned long long bal = __ballot(mv);
+        if (mv) {
+            const int at = c + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
+            map[1 + at] = erow[i];
+            map[1 + 2 * ONB + at] = esrc[i];
+        }
+        c += __popcll(bal);
+    }
+    if (lane == 0) map[0] = c;
 }
 
 // Applies a swap map to columns [c0, c1) and [c2, c3).  Each workgroup owns
@@ -394,15 +435,20 @@ __global__ __launch_bounds__(256) void k_trsm_llu(T *__restrict__ A, long lda, i
         }
     }
     __syncthreads();
-    const int cj = tid & 63, rq = tid >> 6;
+    // each wave owns 16 whole columns (lane = column + 16 * row group), so a
+    // column's steps only depend on LDS writes of the same wave, which the
+    // LDS applies in order: no workgroup barrier inside the sweep
+    const int cj = (tid >> 6) * 16 + (tid & 15), rq = (tid & 63) >> 4;
     {
 #pragma clang fp contract(off)
         for (int c = 0; c < jb - 1; ++c) {
             const T xc = X[c][cj];
             for (int r = c + 1 + rq; r < jb; r += 4) X[r][cj] = X[r][cj] - L[r][c] * xc;
-            __syncthreads();
+            __builtin_amdgcn_wave_barrier();
+            __asm__ __volatile__("" ::: "memory");  // keep step c + 1's reads after these writes
         }
     }
+    __syncthreads();
     for (int e = tid; e < jb * 64; e += 256) {
         const int r = e >> 6, c = e & 63;
         if (c < nc) A[(long)(j0 + r) * lda + cb + c] = X[r][c];

@@ -115,10 +115,15 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
             const int q = fwd ? qi : nq - 1 - qi;
             const int bs = q == 0 ? bs0 : bs1;
             T x = lane < bs ? W[(long)r * ldw + s0 + q * SB + lane] : T(0);
+            // The loops below run all 64 steps, unrolled, so the LDS reads of
+            // the tile are issued ahead of the serial chain.  Steps beyond the
+            // block size change nothing: the staged tiles are zero outside the
+            // block and x is zero on lanes beyond it, so they subtract exact
+            // zeros.
             if (qi == 1) {  // contribution of the sub-block solved first
                 const T xp = fwd ? xq0 : xq1;
-                const int ps = fwd ? bs0 : bs1;
-                for (int c = 0; c < ps; ++c) {
+#pragma unroll
+                for (int c = 0; c < SB; ++c) {
                     const T xc = lane_val(xp, c);
                     const T m = MODE == LOWER_T ? tl[2][c][lane] : tl[2][lane][c];
                     x = x - m * xc;
@@ -128,13 +133,16 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
             const T dl = lane < bs ? d[lane][lane] : T(1);
             const T rd = T(1) / dl;  // used by the FP32 form only
             if (fwd) {
-                for (int c = 0; c < bs; ++c) {
+#pragma unroll
+                for (int c = 0; c < SB; ++c) {
                     if (MODE == LOWER && lane == c) x = diag_div(x, dl, rd);
                     const T xc = lane_val(x, c);
                     if (lane > c) x = x - d[lane][c] * xc;
                 }
             } else {
-                for (int c = bs - 1; c >= 0; --c) {
+#pragma unroll
+                for (int cc = 0; cc < SB; ++cc) {
+                    const int c = SB - 1 - cc;
                     if (lane == c) x = diag_div(x, dl, rd);
                     const T xc = lane_val(x, c);
                     if (lane < c) x = x - (MODE == UPPER ? d[lane][c] : d[c][lane]) * xc;
