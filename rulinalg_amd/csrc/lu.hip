@@ -252,9 +252,22 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
             T mult = T(0);
             if (act) mult = tile[r][j] / piv;
             __syncthreads();
+            static_assert(PTHREADS == PanelRows<T>::value, "one thread per panel row");
+            (void)h;
             if (act) {
-                if (h == 0) tile[r][j] = mult;
-                for (int c = j + 1 + h; c < w; c += PTHREADS / PRT) tile[r][c] = tile[r][c] - mult * prow[c];
+                tile[r][j] = mult;
+                // eight columns per uniform branch, their LDS reads issued
+                // together (a plain loop waits one LDS latency per column)
+#pragma unroll
+                for (int c0 = 0; c0 < PNB; c0 += 8) {
+                    if (c0 + 7 > j && c0 < w) {
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            const int c = c0 + q;
+                            if (c > j && c < w) tile[r][c] = tile[r][c] - mult * prow[c];
+                        }
+                    }
+                }
             }
             __syncthreads();
         }
@@ -443,7 +456,13 @@ __global__ __launch_bounds__(256) void k_trsm_llu(T *__restrict__ A, long lda, i
 #pragma clang fp contract(off)
         for (int c = 0; c < jb - 1; ++c) {
             const T xc = X[c][cj];
-            for (int r = c + 1 + rq; r < jb; r += 4) X[r][cj] = X[r][cj] - L[r][c] * xc;
+            // the (up to 16) rows of this lane's group, unrolled so their LDS
+            // reads are issued together
+#pragma unroll
+            for (int i = 0; i < PNB / 4; ++i) {
+                const int r = c + 1 + rq + 4 * i;
+                if (r < jb) X[r][cj] = X[r][cj] - L[r][c] * xc;
+            }
             __builtin_amdgcn_wave_barrier();
             __asm__ __volatile__("" ::: "memory");  // keep step c + 1's reads after these writes
         }
