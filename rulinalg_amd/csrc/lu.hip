@@ -31,6 +31,7 @@ constexpr int PNB = 64;    // inner panel width
 // workgroup on MI355X, so fewer, taller workgroups make every column cheaper.
 constexpr int PR = 256;
 template <class T> struct PanelRows { static constexpr int value = PR; };
+template <> struct PanelRows<float> { static constexpr int value = 2 * PR; };  // same 133 KB
 constexpr int ONB = 256;   // outer block: depth of the trailing GEMM
 constexpr int PTHREADS = 256;
 
@@ -247,24 +248,32 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
         {
 #pragma clang fp contract(off)
             const T piv = prow[j];
-            const int r = tid & (PRT - 1), h = tid / PRT;
-            const bool act = r < rows && r0 + r > j;
-            T mult = T(0);
-            if (act) mult = tile[r][j] / piv;
+            constexpr int RPT = PRT / PTHREADS;  // panel rows per thread
+            static_assert(RPT * PTHREADS == PRT, "whole rows per thread");
+            T mult[RPT];
+            bool act[RPT];
+#pragma unroll
+            for (int h = 0; h < RPT; ++h) {
+                const int r = tid + h * PTHREADS;
+                act[h] = r < rows && r0 + r > j;
+                mult[h] = act[h] ? tile[r][j] / piv : T(0);
+            }
             __syncthreads();
-            static_assert(PTHREADS == PanelRows<T>::value, "one thread per panel row");
-            (void)h;
-            if (act) {
-                tile[r][j] = mult;
-                // eight columns per uniform branch, their LDS reads issued
-                // together (a plain loop waits one LDS latency per column)
 #pragma unroll
-                for (int c0 = 0; c0 < PNB; c0 += 8) {
-                    if (c0 + 7 > j && c0 < w) {
+            for (int h = 0; h < RPT; ++h) {
+                const int r = tid + h * PTHREADS;
+                if (act[h]) {
+                    tile[r][j] = mult[h];
+                    // eight columns per uniform branch, their LDS reads issued
+                    // together (a plain loop waits one LDS latency per column)
 #pragma unroll
-                        for (int q = 0; q < 8; ++q) {
-                            const int c = c0 + q;
-                            if (c > j && c < w) tile[r][c] = tile[r][c] - mult * prow[c];
+                    for (int c0 = 0; c0 < PNB; c0 += 8) {
+                        if (c0 + 7 > j && c0 < w) {
+#pragma unroll
+                            for (int q = 0; q < 8; ++q) {
+                                const int c = c0 + q;
+                                if (c > j && c < w) tile[r][c] = tile[r][c] - mult[h] * prow[c];
+                            }
                         }
                     }
                 }
