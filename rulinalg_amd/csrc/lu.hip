@@ -34,6 +34,7 @@ template <class T> struct PanelRows { static constexpr int value = PR; };
 template <> struct PanelRows<float> { static constexpr int value = 2 * PR; };  // same 133 KB
 constexpr int ONB = 256;   // outer block: depth of the trailing GEMM
 constexpr int PTHREADS = 256;
+constexpr int BAR_STRIDE = 256;  // barrier counters per panel: top at 0, group i at 32 (i + 1)
 
 template <class T> __device__ __forceinline__ T eps_of();
 template <> __device__ __forceinline__ double eps_of<double>() { return 2.220446049250313e-16; }
@@ -41,7 +42,7 @@ template <> __device__ __forceinline__ float eps_of<float>() { return 1.1920929e
 
 // Workspace layout (bytes): barrier counters, candidate slots, swap map.
 struct LuWork {
-    unsigned *bar;     // one counter per inner panel
+    unsigned *bar;     // BAR_STRIDE barrier counters per inner panel
     void *cval;        // [2][GMAX] T
     int *crow;         // [2][GMAX]
     void *cdat;        // [2][GMAX][PNB] T
@@ -55,7 +56,7 @@ static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t lu_work_bytes(size_t n, size_t elem) {
     size_t panels = (n + PNB - 1) / PNB + 1;
-    return align_up(panels * 4) + align_up(2 * GMAX * elem) + align_up(2 * GMAX * 4) +
+    return align_up(panels * BAR_STRIDE * 4) + align_up(2 * GMAX * elem) + align_up(2 * GMAX * 4) +
            align_up(2 * GMAX * PNB * elem) + align_up(2 * PNB * elem) +
            align_up((1 + 4 * ONB) * 4) + align_up(4);
 }
@@ -64,7 +65,7 @@ static LuWork carve(void *work, size_t n, size_t elem) {
     char *p = static_cast<char *>(work);
     LuWork w;
     size_t panels = (n + PNB - 1) / PNB + 1;
-    w.bar = reinterpret_cast<unsigned *>(p); p += align_up(panels * 4);
+    w.bar = reinterpret_cast<unsigned *>(p); p += align_up(panels * BAR_STRIDE * 4);
     w.cval = p; p += align_up(2 * GMAX * elem);
     w.crow = reinterpret_cast<int *>(p); p += align_up(2 * GMAX * 4);
     w.cdat = p; p += align_up(2 * GMAX * PNB * elem);
@@ -107,16 +108,40 @@ __device__ __forceinline__ void wave_argmax(T &v, int &i) {
     }
 }
 
-// Spin until *bar reaches target.  Bounded: after 2^22 polls the shared
-// timeout flag is raised; every waiter also leaves as soon as it sees the
-// flag, so a barrier that cannot complete ends the kernel (with an error the
-// host reports) instead of hanging it.
-__device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int *timeout) {
+// Grid barrier for the panel's workgroups, the epoch-th of this panel.
+// Two-level arrival: with more than 8 workgroups, workgroup g first meets
+// the others of its group g % 8 (the workgroups a round-robin dispatch puts
+// on one XCD) on a group counter, and only the last of each group arrives at
+// the top counter, so no counter takes more than max(8, G/8) atomic adds
+// per column; everyone then waits on the top counter.  (A flat counter costs
+// about 85 ns per arriving workgroup.)  Release/acquire: each workgroup's
+// release fence precedes its add; the last arriver of a group acquires
+// (through the chain of adds on the group counter) and releases again before
+// its add on the top counter; waiters acquire after reading it.  Bounded:
+// after 2^22 polls the shared timeout flag is raised; every waiter also
+// leaves as soon as it sees the flag, so a barrier that cannot complete ends
+// the kernel (with an error the host reports) instead of hanging it.
+__device__ __forceinline__ void grid_barrier(unsigned *bar, int G, unsigned epoch, int *timeout) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned target;
+        if (G <= 8) {
+            __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            target = (unsigned)G * epoch;
+        } else {
+            const int grp = blockIdx.x & 7;
+            const unsigned ng = (unsigned)((G - grp + 7) >> 3);
+            const unsigned old =
+                __hip_atomic_fetch_add(bar + 32 * (grp + 1), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (old + 1 == ng * epoch) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            target = 8u * epoch;
+        }
         unsigned spins = 0;
         while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if (__hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
@@ -199,7 +224,7 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
             if (tid == 0) { cval[par * G + g] = red_v[0]; crow[par * G + g] = li; }
             if (li != INT_MAX && tid < w) my_cdat[tid] = tile[li - r0][tid];
             if (own_j && tid < w) rowj[par * PNB + tid] = tile[j - r0][tid];
-            grid_barrier(bar, (unsigned)(G * (j + 1)), timeout);
+            grid_barrier(bar, G, (unsigned)(j + 1), timeout);
             // 3. every workgroup reduces all candidates to the same winner
             if (wave == 0) {
                 T v = T(0);
@@ -586,7 +611,7 @@ static int outer_panel(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv,
                        const LuWork &lw, size_t k0, size_t kb) {
     for (size_t j0 = k0; j0 < k0 + kb; j0 += PNB) {
         const size_t jb = k0 + kb - j0 < (size_t)PNB ? k0 + kb - j0 : PNB;
-        RLG_TRY(panel_any<T>(s, a, lda, j0, n - j0, jb, ipiv, info, n, lw, lw.bar + j0 / PNB));
+        RLG_TRY(panel_any<T>(s, a, lda, j0, n - j0, jb, ipiv, info, n, lw, lw.bar + (j0 / PNB) * BAR_STRIDE));
         // inner swaps on the other columns of the outer block
         RLG_TRY(laswp<T>(s, a, lda, ipiv, j0, jb, k0, j0, j0 + jb, k0 + kb, lw));
         if (j0 + jb < k0 + kb) {
@@ -615,7 +640,7 @@ static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ip
                           void *work, UpdateFn<T> big_update) {
     LuWork lw = carve(work, n, sizeof(T));
     const size_t panels = (n + PNB - 1) / PNB + 1;
-    RLG_TRY(fill_u32(lw.bar, panels, 0u, s));
+    RLG_TRY(fill_u32(lw.bar, panels * BAR_STRIDE, 0u, s));
     RLG_TRY(fill_u32(reinterpret_cast<uint32_t *>(lw.timeout), 1, 0u, s));
     RLG_TRY(fill_i64(info, 1, (int64_t)n, s));
     if (n == 0) return RLG_OK;
