@@ -43,7 +43,7 @@ def parse():
     p.add_argument("--n", type=int, default=16384)
     p.add_argument("--nrhs", type=int, default=1)
     p.add_argument("--kappa", type=float, default=1e4)
-    p.add_argument("--prec", default="f16", choices=sorted(PREC))
+    p.add_argument("--prec", default="f32", choices=sorted(PREC))
     p.add_argument("--plain-steps", type=int, default=None)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extras", action="store_true")
@@ -262,7 +262,7 @@ def main():
         "metric": METRIC, "value": round(value, 4), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(mixed_ms / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": f"{args.prec} factor + f32 panels, f64 refinement", "data": "synthetic",
+        "dtype": (f"{args.prec} factor + f32 panels" if args.prec != "f32" else "f32 factor") + ", f64 refinement", "data": "synthetic",
         "config": {"workload": f"refined mixed-precision LU solve, A = U diag(s) V^T, n={n}, "
                                f"kappa={args.kappa:g}, {nrhs} RHS", "n": n, "nrhs": nrhs,
                    "kappa": args.kappa, "factor_prec": args.prec, "tol": 1e-13},
@@ -302,7 +302,7 @@ def main():
         B32 = torch.randn(32, n, dtype=torch.float64, device=dev, generator=g)
         X32 = torch.empty_like(B32)
         torch.cuda.synchronize()
-        for prec in ("f32", args.prec):
+        for prec in ("f32", "f16"):
             mixed_once(A, B32, X32, 32, prec)
             variants[f"mixed_{prec}_32rhs_kappa1e4"] = mixed_once(A, B32, X32, 32, prec)
         # plain FP64 LU + 16 right-hand sides (factor and solve)
