@@ -34,7 +34,11 @@ template <class T> struct PanelRows { static constexpr int value = PR; };
 template <> struct PanelRows<float> { static constexpr int value = 2 * PR; };  // same 133 KB
 constexpr int ONB = 256;   // outer block: depth of the trailing GEMM
 constexpr int PTHREADS = 256;
-constexpr int BAR_STRIDE = 256;  // barrier counters per panel: top at 0, group i at 32 (i + 1)
+// Barrier counters per panel: top at 0, group i (0..7) at 32 (i + 1), i.e.
+// up to index 256, so each panel's counters span 257 entries (128-byte
+// spacing); the next panel's start at BAR_STRIDE.
+constexpr int BAR_STRIDE = 512;
+static_assert(32 * 8 < BAR_STRIDE, "group counters must stay inside the panel's block");
 
 template <class T> __device__ __forceinline__ T eps_of();
 template <> __device__ __forceinline__ double eps_of<double>() { return 2.220446049250313e-16; }

@@ -209,7 +209,7 @@ def test_lu_bit_exact_single_panel(n):
     assert np.array_equal(lu.packed(), lu_or)
 
 
-@pytest.mark.parametrize("n", [65, 127, 128, 129, 200, 255, 256, 257, 300, 513, 777])
+@pytest.mark.parametrize("n", [65, 127, 128, 129, 200, 255, 256, 257, 300, 513, 777, 2304])
 def test_lu_vs_oracle(n):
     a = wl.gaussian(n, n, 2000 + n)
     lu_or, perm_or, ipiv_or, margin = orc.lu_decompose(a, with_margin=True)
@@ -228,6 +228,20 @@ def test_lu_vs_oracle(n):
     assert eta(a, x, b) <= 1e-13
     x_or = orc.lu_solve(lu_or, perm_or, b)
     assert np.abs(x - x_or).max() / np.abs(x_or).max() <= fwd_tol(a)
+
+
+def test_lu_f32_and_mixed_wide_panels():
+    """Order 4608: the first FP32 panels span nine 512-row workgroups, so the
+    panel barrier takes its two-level form (FP64 reaches it at 2304, above)."""
+    n = 4608
+    a = wl.gaussian(n, n, 4608)
+    b = wl.gaussian(1, n, 4609)[0]
+    lu = rl.PartialPivLu.decompose(rl.Matrix(a.astype(np.float32)))
+    x = lu.solve(rl.Vector(b.astype(np.float32))).data().astype(np.float64)
+    assert eta(a, x, b) <= 2e-5
+    xm, rep = rl.solve_mixed(a, b, rl.MixedOptions(factor_prec=_lib.PREC_F32))
+    assert rep["converged"] == 1 and rep["fell_back"] == 0
+    assert eta(a, xm.reshape(-1), b) <= 1e-13
 
 
 def test_lu_multi_rhs():
