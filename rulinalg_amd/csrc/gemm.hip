@@ -18,32 +18,12 @@
 // Fragments hold 8 consecutive k per lane, so both stages are k-contiguous
 // ([i][k] and [j][k]).
 #include "kernels.hpp"
+#include "mfma.hpp"
 
 namespace rlg {
 
-typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
-
-template <class T> struct Mfma;
-// Accumulator layouts of the 16 x 16 result: for the FP32 instruction lane l,
-// register r holds D[4*(l/16) + r][l%16]; for the FP64 instruction it holds
-// D[(l/16) + 4*r][l%16].
-template <> struct Mfma<double> {
-    using acc = d4;
-    static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
-    static __device__ __forceinline__ acc op(double a, double b, acc c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-};
-template <> struct Mfma<float> {
-    using acc = f4;
-    static __device__ __forceinline__ int row(int lk, int r) { return lk * 4 + r; }
-    static __device__ __forceinline__ acc op(float a, float b, acc c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-};
 
 constexpr int GBM = 128, GBN = 128, GBK = 16, GTHREADS = 256;
 

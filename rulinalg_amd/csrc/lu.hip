@@ -9,9 +9,9 @@
 //
 // The panel kernel keeps the whole (n - k) x 64 panel resident in LDS,
 // spread over G workgroups of PR rows each.  Every column costs one
-// grid-wide barrier: each workgroup publishes its local pivot candidate
-// (|value|, row, and the full 64-wide row) and the content of the current
-// diagonal row; after the barrier every workgroup reduces the G candidates
+// exchange: each workgroup publishes its local pivot candidate (|value|,
+// row, and the full 64-wide row) and the content of the current diagonal
+// row as tagged words; every workgroup reads the G candidates, reduces them
 // to the same winner, performs its part of the row swap, and eliminates its
 // rows.  The pivot rule is the reference's: the first row of maximal
 // magnitude at or below the diagonal (strict `>` scan, lu.rs:170-178); the
@@ -22,58 +22,70 @@
 #include <type_traits>
 
 #include "kernels.hpp"
+#include "mfma.hpp"
+#include "wave.hpp"
 
 namespace rlg {
 
 constexpr int PNB = 64;    // inner panel width
 // Panel rows per workgroup (LDS-resident): 256 (133 KB of LDS in FP64, one
-// workgroup per CU).  A grid barrier costs about 85 ns per participating
-// workgroup on MI355X, so fewer, taller workgroups make every column cheaper.
+// workgroup per CU, so every workgroup of a panel is resident at once).
 constexpr int PR = 256;
 template <class T> struct PanelRows { static constexpr int value = PR; };
 template <> struct PanelRows<float> { static constexpr int value = 2 * PR; };  // same 133 KB
 constexpr int ONB = 256;   // outer block: depth of the trailing GEMM
 constexpr int PTHREADS = 256;
-// Barrier counters per panel: top at 0, group i (0..7) at 32 (i + 1), i.e.
-// up to index 256, so each panel's counters span 257 entries (128-byte
-// spacing); the next panel's start at BAR_STRIDE.
-constexpr int BAR_STRIDE = 512;
-static_assert(32 * 8 < BAR_STRIDE, "group counters must stay inside the panel's block");
+// At most this many workgroups per panel: one per CU of the device and never
+// more than the exchange area holds.
+constexpr int GMAX = 256;
 
 template <class T> __device__ __forceinline__ T eps_of();
 template <> __device__ __forceinline__ double eps_of<double>() { return 2.220446049250313e-16; }
 template <> __device__ __forceinline__ float eps_of<float>() { return 1.1920929e-7f; }
 
-// Workspace layout (bytes): barrier counters, candidate slots, swap map.
-struct LuWork {
-    unsigned *bar;     // BAR_STRIDE barrier counters per inner panel
-    void *cval;        // [2][GMAX] T
-    int *crow;         // [2][GMAX]
-    void *cdat;        // [2][GMAX][PNB] T
-    void *rowj;        // [2][PNB] T
-    int *map;          // [1 + 2 * 2 * ONB]: count, dst[], src[]
-    int *timeout;      // barrier timeout flag
+// Exchange words of the panel workgroups.  A 64-bit word carries 32 bits of
+// payload in its low half and, in its high half, the tag of the column it
+// belongs to: global column + 1, 0 = never written in this factorization.
+// Words are written and read with agent-scope atomics, and a reader polls
+// until every word it needs carries the current tag, so a value is never
+// used before it is complete and no counter, fence or barrier is involved.
+// A slot holds a header (candidate row, candidate |value|) and a 64-wide row;
+// a value takes one word in FP32 and two (low, high half) in FP64.
+typedef unsigned long long xword;
+constexpr int XHDR = 4;  // header words: row, |v| (1 or 2 words), spare
+template <class T> struct XSlot {
+    static constexpr int VW = sizeof(T) / 4;                      // words per value
+    static constexpr int words = (XHDR + PNB * VW + 15) & ~15;    // 128-byte multiple
 };
-constexpr int GMAX = 1024;
+// Slots per column parity: one per workgroup, then the diagonal-row slot.
+constexpr int XPAR = GMAX + 1;
+
+// Workspace layout (bytes): exchange slots, inverses of the unit-lower
+// diagonal blocks of the current outer block, swap map, timeout flag.
+struct LuWork {
+    xword *xch;        // [2][XPAR][XSlot<T>::words]
+    void *linv;        // [ONB / PNB][PNB][PNB] T
+    int *map;          // [1 + 2 * 2 * ONB]: count, dst[], src[]
+    int *timeout;      // exchange timeout flag
+};
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+static size_t xch_words(size_t elem) {
+    return 2 * (size_t)XPAR * (elem == 8 ? XSlot<double>::words : XSlot<float>::words);
+}
 
 size_t lu_work_bytes(size_t n, size_t elem) {
-    size_t panels = (n + PNB - 1) / PNB + 1;
-    return align_up(panels * BAR_STRIDE * 4) + align_up(2 * GMAX * elem) + align_up(2 * GMAX * 4) +
-           align_up(2 * GMAX * PNB * elem) + align_up(2 * PNB * elem) +
+    (void)n;
+    return align_up(xch_words(elem) * sizeof(xword)) + align_up((size_t)ONB * PNB * elem) +
            align_up((1 + 4 * ONB) * 4) + align_up(4);
 }
 
 static LuWork carve(void *work, size_t n, size_t elem) {
+    (void)n;
     char *p = static_cast<char *>(work);
     LuWork w;
-    size_t panels = (n + PNB - 1) / PNB + 1;
-    w.bar = reinterpret_cast<unsigned *>(p); p += align_up(panels * BAR_STRIDE * 4);
-    w.cval = p; p += align_up(2 * GMAX * elem);
-    w.crow = reinterpret_cast<int *>(p); p += align_up(2 * GMAX * 4);
-    w.cdat = p; p += align_up(2 * GMAX * PNB * elem);
-    w.rowj = p; p += align_up(2 * PNB * elem);
+    w.xch = reinterpret_cast<xword *>(p); p += align_up(xch_words(elem) * sizeof(xword));
+    w.linv = p; p += align_up((size_t)ONB * PNB * elem);
     w.map = reinterpret_cast<int *>(p); p += align_up((1 + 4 * ONB) * 4);
     w.timeout = reinterpret_cast<int *>(p);
     return w;
@@ -102,78 +114,117 @@ __device__ __forceinline__ bool better(T v2, int i2, T v1, int i1) {
     return i2 < i1;
 }
 
+// (v, i) and a payload := the wave's best under `better`, in every lane.
 template <class T>
-__device__ __forceinline__ void wave_argmax(T &v, int &i) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        T ov = __shfl_xor(v, off, 64);
-        int oi = __shfl_xor(i, off, 64);
-        if (better(ov, oi, v, i)) { v = ov; i = oi; }
-    }
+__device__ __forceinline__ void wave_argmax(T &v, int &i, int &pay) {
+    wave_argmax_dpp(v, i, pay, [](T v2, int i2, T v1, int i1) { return better(v2, i2, v1, i1); });
 }
 
-// Grid barrier for the panel's workgroups, the epoch-th of this panel.
-// Two-level arrival: with more than 8 workgroups, workgroup g first meets
-// the others of its group g % 8 (the workgroups a round-robin dispatch puts
-// on one XCD) on a group counter, and only the last of each group arrives at
-// the top counter, so no counter takes more than max(8, G/8) atomic adds
-// per column; everyone then waits on the top counter.  (A flat counter costs
-// about 85 ns per arriving workgroup.)  Release/acquire: each workgroup's
-// release fence precedes its add; the last arriver of a group acquires
-// (through the chain of adds on the group counter) and releases again before
-// its add on the top counter; waiters acquire after reading it.  Bounded:
-// after 2^22 polls the shared timeout flag is raised; every waiter also
-// leaves as soon as it sees the flag, so a barrier that cannot complete ends
-// the kernel (with an error the host reports) instead of hanging it.
-__device__ __forceinline__ void grid_barrier(unsigned *bar, int G, unsigned epoch, int *timeout) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        unsigned target;
-        if (G <= 8) {
-            __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            target = (unsigned)G * epoch;
-        } else {
-            const int grp = blockIdx.x & 7;
-            const unsigned ng = (unsigned)((G - grp + 7) >> 3);
-            const unsigned old =
-                __hip_atomic_fetch_add(bar + 32 * (grp + 1), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (old + 1 == ng * epoch) {
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            target = 8u * epoch;
-        }
-        unsigned spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-            if (++spins > (1u << 22)) {
+__device__ __forceinline__ unsigned xtag(xword w) { return (unsigned)(w >> 32); }
+__device__ __forceinline__ void xput(xword *p, unsigned tag, unsigned v) {
+    __hip_atomic_store(p, ((xword)tag << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ xword xget(const xword *p) {
+    return __hip_atomic_load(const_cast<xword *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void xput_val(xword *p, unsigned tag, float v) {
+    xput(p, tag, __float_as_uint(v));
+}
+__device__ __forceinline__ void xput_val(xword *p, unsigned tag, double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    xput(p, tag, (unsigned)b);
+    xput(p + 1, tag, (unsigned)(b >> 32));
+}
+// The value at p; ok = every word of it carries `tag`.
+template <class T> __device__ __forceinline__ T xget_val(const xword *p, unsigned tag, bool &ok);
+template <> __device__ __forceinline__ float xget_val<float>(const xword *p, unsigned tag, bool &ok) {
+    const xword a = xget(p);
+    ok = xtag(a) == tag;
+    return __uint_as_float((unsigned)a);
+}
+template <> __device__ __forceinline__ double xget_val<double>(const xword *p, unsigned tag, bool &ok) {
+    const xword a = xget(p), b = xget(p + 1);
+    ok = xtag(a) == tag && xtag(b) == tag;
+    return __longlong_as_double((long long)(((b & 0xffffffffull) << 32) | (a & 0xffffffffull)));
+}
+
+// One round of a bounded wave-wide wait.  Returns false once the timeout
+// flag is up (checked every 256 rounds) or after 2^22 rounds, when it raises
+// the flag itself: an exchange that cannot complete then ends the kernel,
+// and the host reports the flag, instead of hanging.  The result is uniform
+// over the wave.
+__device__ __forceinline__ bool xwait(unsigned &spins, int *timeout) {
+    if ((++spins & 255u) == 0) {
+        const int up = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (up) return false;
+        if (spins >= (1u << 22)) {
+            if ((threadIdx.x & 63) == 0)
                 __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
+            return false;
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_s_sleep(1);
+    return true;
 }
 
 // Factor the panel A[k:k+m, k:k+w] (w <= PNB) in place.
+//
+// Workgroup g holds rows [g PRT, (g+1) PRT) of the panel in LDS; thread t
+// owns the tile rows t + h PTHREADS and does every local step on them, so a
+// column needs only the two workgroup barriers around wave 0's exchange.
+// Column j:
+//   1. every thread finds the first maximum |a[r][j]| over its rows r >= j;
+//   2. wave 0 reduces the wave candidates to the workgroup's, publishes it
+//      (row index, |value| and the whole row) and, if it holds row j, row j;
+//      reads all G candidates and picks the same winner as every other
+//      workgroup; fetches the winning row unless it is its own and, if it
+//      holds the winning row but not row j, row j; swaps the rows it holds
+//      and puts the pivot row in LDS;
+//   3. every thread eliminates its rows below j.
+// The pivot indices stay in LDS until the end: a global store before a
+// workgroup barrier would make the barrier wait for its completion.
+
+// Tile rows are padded to a 16-byte multiple: 16-byte aligned rows let the
+// row sweeps use 16-byte LDS accesses, and the odd multiple of 16 bytes
+// spreads a column over the banks.
+template <class T> struct TilePad { static constexpr int value = 16 / sizeof(T); };
+template <class T> constexpr int TW = PNB + TilePad<T>::value;
+
+// a[r][c] = a[r][c] - mult * prow[c] for c > j, as one straight sweep over
+// the columns from C0 on (C0 <= j + 1): every column is read, and written
+// back unchanged where c <= j, so that no column costs a branch (a branch
+// per column serializes one LDS round trip per column) and the reads go out
+// back to back.  Columns w..63 of the tile and of prow hold zeros and stay
+// zero.  The caller writes column j afterwards.
+template <class T, int C0>
+__device__ __forceinline__ void elim_row(T (*__restrict__ tile)[TW<T>], const T *__restrict__ prow,
+                                         int r, int j, T mult) {
+#pragma clang fp contract(off)
+    T x[PNB - C0];
+#pragma unroll
+    for (int c = C0; c < PNB; ++c) x[c - C0] = tile[r][c];
+#pragma unroll
+    for (int c = C0; c < PNB; ++c) {
+        const T y = x[c - C0] - mult * prow[c];
+        tile[r][c] = c > j ? y : x[c - C0];
+    }
+}
+
 template <class T>
 __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long lda, int k, int m,
                                                        int w, int64_t *__restrict__ ipiv,
                                                        int64_t *__restrict__ info, int n,
-                                                       unsigned *bar, T *cval, int *crow,
-                                                       T *cdat, T *rowj, int *timeout) {
+                                                       xword *__restrict__ xch, int *timeout) {
     constexpr int PRT = PanelRows<T>::value;
-    __shared__ T tile[PRT][PNB + 1];
-    __shared__ T prow[PNB];
+    constexpr int RPT = PRT / PTHREADS;  // panel rows per thread
+    static_assert(RPT * PTHREADS == PRT, "whole rows per thread");
+    constexpr int VW = XSlot<T>::VW, XS = XSlot<T>::words;
+    __shared__ __attribute__((aligned(16))) T tile[PRT][TW<T>];
+    __shared__ __attribute__((aligned(16))) T prow[PNB];
     __shared__ T red_v[PTHREADS / 64];
     __shared__ int red_i[PTHREADS / 64];
-    __shared__ int s_piv;
+    __shared__ int s_piv[PNB];
 
     const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
     const int r0 = g * PRT;
@@ -191,224 +242,217 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = q * PTHREADS + tid, r = e / PNB, c = e % PNB;
-            if (r < rows && c < w) tile[r][c] = v[q];
+            if (r < rows) tile[r][c] = v[q];  // columns w..63 zero
         }
     }
     __syncthreads();
 
+    bool live = true;  // wave 0: false once an exchange has timed out
+    int small = w;     // wave 0: first column whose pivot is below epsilon
     for (int j = 0; j < w; ++j) {
-        const int par = j & 1;
-        // 1. local candidate: first maximum of |tile[r][j]| over owned rows >= j
+        // 1. this thread's candidate over its rows >= j
         T bv = T(0);
         int bi = INT_MAX;
-        for (int r = tid; r < rows; r += PTHREADS) {
-            const int gr = r0 + r;
-            if (gr >= j) {
-                T v = fabs(tile[r][j]);
-                if (better(v, gr, bv, bi)) { bv = v; bi = gr; }
+#pragma unroll
+        for (int h = 0; h < RPT; ++h) {
+            const int r = tid + h * PTHREADS;
+            if (r < rows && r0 + r >= j) {
+                const T v = fabs(tile[r][j]);
+                if (better(v, r0 + r, bv, bi)) { bv = v; bi = r0 + r; }
             }
         }
-        wave_argmax(bv, bi);
+        {
+            int unused = 0;
+            wave_argmax(bv, bi, unused);
+        }
         if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
         __syncthreads();
-        if (tid == 0) {
+
+        if (wave == 0) {
+            // 2. the workgroup's candidate (uniform over the wave)
+#pragma unroll
             for (int q = 1; q < PTHREADS / 64; ++q)
                 if (better(red_v[q], red_i[q], bv, bi)) { bv = red_v[q]; bi = red_i[q]; }
-            red_v[0] = bv;
-            red_i[0] = bi;
-        }
-        __syncthreads();
-        const int li = red_i[0];
-        const bool own_j = (j >= r0 && j < r0 + rows);
-
-        int p;
-        if (G > 1) {
-            // 2. publish candidate row and the diagonal row, then meet
-            T *my_cdat = cdat + ((long)par * G + g) * PNB;
-            if (tid == 0) { cval[par * G + g] = red_v[0]; crow[par * G + g] = li; }
-            if (li != INT_MAX && tid < w) my_cdat[tid] = tile[li - r0][tid];
-            if (own_j && tid < w) rowj[par * PNB + tid] = tile[j - r0][tid];
-            grid_barrier(bar, G, (unsigned)(j + 1), timeout);
-            // 3. every workgroup reduces all candidates to the same winner
-            if (wave == 0) {
+            const bool own_j = j >= r0 && j < r0 + rows;
+            // lane c: column c of the candidate row and of row j
+            T pv = (lane < w && bi != INT_MAX) ? tile[bi - r0][lane] : T(0);
+            T oj = (lane < w && own_j) ? tile[j - r0][lane] : T(0);
+            int p = bi;
+            T pabs = bv;
+            if (G > 1 && live) {
+                const unsigned tag = (unsigned)(k + j + 1);
+                xword *base = xch + (size_t)(j & 1) * XPAR * XS;
+                xword *mine = base + (size_t)g * XS, *diag = base + (size_t)GMAX * XS;
+                if (lane < w) {
+                    if (bi != INT_MAX) xput_val(mine + XHDR + lane * VW, tag, pv);
+                    if (own_j) xput_val(diag + XHDR + lane * VW, tag, oj);
+                }
+                if (lane == 0) {
+                    xput(mine, tag, (unsigned)bi);
+                    xput_val(mine + 1, tag, bv);
+                }
+                // every workgroup's candidate, 64 slots per round
                 T v = T(0);
                 int i = INT_MAX, gw = 0;
-                for (int q = lane; q < G; q += 64) {
-                    T qv = cval[par * G + q];
-                    int qi = crow[par * G + q];
-                    if (better(qv, qi, v, i)) { v = qv; i = qi; gw = q; }
+                for (int q0 = 0; q0 < G && live; q0 += 64) {
+                    const int q = q0 + lane;
+                    const xword *h = base + (size_t)q * XS;
+                    T qv = T(0);
+                    int qi = INT_MAX;
+                    for (unsigned spins = 0;;) {
+                        bool ok = true;
+                        if (q < G) {
+                            const xword hr = xget(h);
+                            qv = xget_val<T>(h + 1, tag, ok);
+                            qi = (int)(unsigned)hr;
+                            ok = ok && xtag(hr) == tag;
+                        }
+                        if (__all(ok)) break;
+                        if (!xwait(spins, timeout)) { live = false; break; }
+                    }
+                    if (q < G && better(qv, qi, v, i)) { v = qv; i = qi; gw = q; }
                 }
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-                    T ov = __shfl_xor(v, off, 64);
-                    int oi = __shfl_xor(i, off, 64);
-                    int og = __shfl_xor(gw, off, 64);
-                    if (better(ov, oi, v, i)) { v = ov; i = oi; gw = og; }
-                }
-                if (lane == 0) s_piv = i;
-                // winner's row data
-                if (lane < w) prow[lane] = cdat[((long)par * G + gw) * PNB + lane];
-            }
-            __syncthreads();
-            p = s_piv;
-            // 4. swap rows j and p in the owning tiles (old row j comes from
-            //    the published copy when it lives in another workgroup)
-            if (p != j && tid < w) {
-                const T old_j = own_j ? tile[j - r0][tid] : rowj[par * PNB + tid];
-                if (own_j) tile[j - r0][tid] = prow[tid];
-                if (p >= r0 && p < r0 + rows) tile[p - r0][tid] = old_j;
-            }
-        } else {
-            p = li;
-            if (tid < w) {
-                const T vp = tile[p][tid];
-                const T vj = tile[j][tid];
-                prow[tid] = vp;
-                if (p != j) { tile[j][tid] = vp; tile[p][tid] = vj; }
-            }
-        }
-        __syncthreads();
-        if (g == 0 && tid == 0) {
-            ipiv[k + j] = (int64_t)(k + p);
-            if (fabs(prow[j]) < eps_of<T>() && *info == (int64_t)n) *info = k + j;
-        }
-
-        // 5. eliminate: mult = a[i][j] / piv; a[i][c] = a[i][c] - mult * piv_row[c]
-        {
-#pragma clang fp contract(off)
-            const T piv = prow[j];
-            constexpr int RPT = PRT / PTHREADS;  // panel rows per thread
-            static_assert(RPT * PTHREADS == PRT, "whole rows per thread");
-            T mult[RPT];
-            bool act[RPT];
-#pragma unroll
-            for (int h = 0; h < RPT; ++h) {
-                const int r = tid + h * PTHREADS;
-                act[h] = r < rows && r0 + r > j;
-                mult[h] = act[h] ? tile[r][j] / piv : T(0);
-            }
-            __syncthreads();
-#pragma unroll
-            for (int h = 0; h < RPT; ++h) {
-                const int r = tid + h * PTHREADS;
-                if (act[h]) {
-                    tile[r][j] = mult[h];
-                    // eight columns per uniform branch, their LDS reads issued
-                    // together (a plain loop waits one LDS latency per column)
-#pragma unroll
-                    for (int c0 = 0; c0 < PNB; c0 += 8) {
-                        if (c0 + 7 > j && c0 < w) {
-#pragma unroll
-                            for (int q = 0; q < 8; ++q) {
-                                const int c = c0 + q;
-                                if (c > j && c < w) tile[r][c] = tile[r][c] - mult[h] * prow[c];
+                wave_argmax(v, i, gw);
+                if (live) {
+                    p = i;
+                    pabs = v;
+                    const bool need_row = gw != g;
+                    const bool need_j = p != j && p >= r0 && p < r0 + rows && !own_j;
+                    if (need_row || need_j) {
+                        for (unsigned spins = 0;;) {
+                            bool ok = true, okj = true;
+                            if (lane < w) {
+                                if (need_row) {
+                                    const T x = xget_val<T>(base + (size_t)gw * XS + XHDR + lane * VW, tag, ok);
+                                    if (ok) pv = x;
+                                }
+                                if (need_j) {
+                                    const T x = xget_val<T>(diag + XHDR + lane * VW, tag, okj);
+                                    if (okj) oj = x;
+                                }
                             }
+                            if (__all(ok && okj)) break;
+                            if (!xwait(spins, timeout)) { live = false; break; }
                         }
                     }
                 }
+                // after a timeout the result is discarded: keep indices in range
+                if (!live) { p = bi; pabs = bv; }
             }
-            __syncthreads();
+            // swap rows j and p where this workgroup holds them (lanes w..63
+            // carry zeros on both sides)
+            if (p != j) {
+                if (own_j) tile[j - r0][lane] = pv;
+                if (p >= r0 && p < r0 + rows) tile[p - r0][lane] = oj;
+            }
+            prow[lane] = pv;
+            if (lane == 0) s_piv[j] = p;
+            if (pabs < eps_of<T>() && small == w) small = j;
+        }
+        __syncthreads();
+
+        // 3. eliminate: mult = a[i][j] / piv; a[i][c] = a[i][c] - mult * piv_row[c],
+        //    swept from the 8-column chunk that holds column j + 1
+        {
+#pragma clang fp contract(off)
+            const T piv = prow[j];
+#pragma unroll
+            for (int h = 0; h < RPT; ++h) {
+                const int r = tid + h * PTHREADS;
+                if (r < rows && r0 + r > j) {
+                    const T mult = tile[r][j] / piv;
+                    switch ((j + 1) >> 3) {
+                        case 0: elim_row<T, 0>(tile, prow, r, j, mult); break;
+                        case 1: elim_row<T, 8>(tile, prow, r, j, mult); break;
+                        case 2: elim_row<T, 16>(tile, prow, r, j, mult); break;
+                        case 3: elim_row<T, 24>(tile, prow, r, j, mult); break;
+                        case 4: elim_row<T, 32>(tile, prow, r, j, mult); break;
+                        case 5: elim_row<T, 40>(tile, prow, r, j, mult); break;
+                        case 6: elim_row<T, 48>(tile, prow, r, j, mult); break;
+                        case 7: elim_row<T, 56>(tile, prow, r, j, mult); break;
+                        default: break;  // j = 63: nothing right of the pivot
+                    }
+                    tile[r][j] = mult;
+                }
+            }
         }
     }
+    __syncthreads();
 
-    for (int e = tid; e < rows * w; e += PTHREADS) {
-        const int r = e / w, c = e - r * w;
-        A[(long)(k + r0 + r) * lda + k + c] = tile[r][c];
+    if (g == 0) {
+        if (tid < w) ipiv[k + tid] = (int64_t)k + s_piv[tid];
+        if (tid == 0 && small < w && *info == (int64_t)n) *info = k + small;
+    }
+    constexpr int PER = PRT * PNB / PTHREADS;
+#pragma unroll 8
+    for (int q = 0; q < PER; ++q) {
+        const int e = q * PTHREADS + tid, r = e / PNB, c = e % PNB;
+        if (r < rows && c < w) A[(long)(k + r0 + r) * lda + k + c] = tile[r][c];
     }
 }
 
 // ---------------------------------------------------------------- row swaps
 
 // Net effect of the swaps ipiv[k0 .. k0+cnt) as a list of (dst, src) rows:
-// after the swaps, row dst holds what row src held before.  One wave walks
-// the swaps in order with every table in registers (entry e of a table at
-// lane e % 64, slot e / 64, read with readlane): `cont` holds what position
-// k0 + e now contains; rows outside [k0, k0+cnt) that get touched are kept
-// in (erow, esrc) and found with a ballot.  The list is then written with a
-// ballot prefix count.
-constexpr int SMS = ONB / 64;  // table slots per lane
-
-__device__ __forceinline__ int tab_get(const int (&t)[SMS], int e) {
-    int v = t[0];
-#pragma unroll
-    for (int i = 1; i < SMS; ++i)
-        if ((e >> 6) == i) v = t[i];
-    return __builtin_amdgcn_readlane(v, e & 63);
-}
-__device__ __forceinline__ void tab_set(int (&t)[SMS], int e, int v) {
-    if ((int)threadIdx.x == (e & 63)) {
-#pragma unroll
-        for (int i = 0; i < SMS; ++i)
-            if ((e >> 6) == i) t[i] = v;
-    }
+// after the swaps, row dst holds what row src held before.  Swap s exchanges
+// rows k0+s and p_s >= k0+s, so row k0+s is final after swap s and then
+// holds what row p_s held just before it.  Before swap t, a row x >= k0+t
+// holds what row k0+s held before swap s, for the last s < t with p_s = x,
+// and its own content if there is none.  Following that rule back is one
+// downward scan of the swaps from t-1, so every thread resolves its rows on
+// its own: thread i the final content of row k0+i and, when swap i is the
+// last to touch a row p_i beyond the block, that row's (what row k0+i held
+// before swap i).  The list (block rows first) is written with ballot
+// prefix counts.
+__device__ __forceinline__ int swap_chase(const int *P, int k0, int x, int t) {
+    for (int j = t - 1; j >= 0; --j)
+        if (P[j] == x) x = k0 + j;
+    return x;
 }
 
-__global__ __launch_bounds__(64) void k_swap_map(const int64_t *__restrict__ ipiv, int k0, int cnt,
-                                                 int *__restrict__ map) {
-    const int lane = threadIdx.x;
-    int cont[SMS], pv[SMS], erow[SMS], esrc[SMS];
-#pragma unroll
-    for (int i = 0; i < SMS; ++i) {
-        const int e = i * 64 + lane;
-        cont[i] = k0 + e;
-        pv[i] = e < cnt ? (int)ipiv[k0 + e] : k0 + e;
-        erow[i] = -1;
-        esrc[i] = -1;
-    }
-    int n_ext = 0;  // uniform
-    for (int s = 0; s < cnt; ++s) {
-        const int p = tab_get(pv, s);
-        if (p == k0 + s) continue;
-        if (p < k0 + cnt) {
-            const int a = tab_get(cont, s), b = tab_get(cont, p - k0);
-            tab_set(cont, s, b);
-            tab_set(cont, p - k0, a);
-        } else {
-            int idx = -1;
-#pragma unroll
-            for (int i = 0; i < SMS; ++i) {
-                const unsigned long long bal = __ballot(erow[i] == p);
-                if (idx < 0 && bal) idx = i * 64 + __ffsll((long long)bal) - 1;
-            }
-            if (idx < 0) {
-                idx = n_ext++;
-                tab_set(erow, idx, p);
-                tab_set(esrc, idx, p);
-            }
-            const int a = tab_get(cont, s), e = tab_get(esrc, idx);
-            tab_set(cont, s, e);
-            tab_set(esrc, idx, a);
+__global__ __launch_bounds__(ONB) void k_swap_map(const int64_t *__restrict__ ipiv, int k0, int cnt,
+                                                  int *__restrict__ map) {
+    __shared__ int P[ONB];
+    __shared__ int wcount[2][ONB / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (t < cnt) P[t] = (int)ipiv[k0 + t];
+    __syncthreads();
+    int src_in = k0 + t, dst_out = -1, src_out = -1;
+    if (t < cnt) {
+        const int pt = P[t];
+        src_in = swap_chase(P, k0, pt, t);
+        if (pt >= k0 + cnt) {
+            bool last = true;
+            for (int j = t + 1; j < cnt; ++j) last = last && P[j] != pt;
+            if (last) { dst_out = pt; src_out = swap_chase(P, k0, k0 + t, t); }
         }
     }
-    // the list: positions whose content changed, then touched outside rows
-    int c = 0;
+    const bool mv_in = t < cnt && src_in != k0 + t;
+    const bool mv_out = dst_out >= 0 && src_out != dst_out;
+    const unsigned long long b_in = __ballot(mv_in), b_out = __ballot(mv_out);
+    if (lane == 0) { wcount[0][wave] = __popcll(b_in); wcount[1][wave] = __popcll(b_out); }
+    __syncthreads();
+    int off_in = 0, tot_in = 0, off_out = 0, tot_out = 0;
 #pragma unroll
-    for (int i = 0; i < SMS; ++i) {
-        const int e = i * 64 + lane;
-        const bool mv = e < cnt && cont[i] != k0 + e;
-        const unsigned long long bal = __ballot(mv);
-        if (mv) {
-            const int at = c + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                              __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
-            map[1 + at] = k0 + e;
-            map[1 + 2 * ONB + at] = cont[i];
-        }
-        c += __popcll(bal);
+    for (int q = 0; q < ONB / 64; ++q) {
+        if (q < wave) { off_in += wcount[0][q]; off_out += wcount[1][q]; }
+        tot_in += wcount[0][q];
+        tot_out += wcount[1][q];
     }
-#pragma unroll
-    for (int i = 0; i < SMS; ++i) {
-        const int e = i * 64 + lane;
-        const bool mv = e < n_ext && esrc[i] != erow[i];
-        const unsigned long long bal = __ballot(mv);
-        if (mv) {
-            const int at = c + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                              __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
-            map[1 + at] = erow[i];
-            map[1 + 2 * ONB + at] = esrc[i];
-        }
-        c += __popcll(bal);
+    if (mv_in) {
+        const int at = off_in + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b_in >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((unsigned)b_in, 0));
+        map[1 + at] = k0 + t;
+        map[1 + 2 * ONB + at] = src_in;
     }
-    if (lane == 0) map[0] = c;
+    if (mv_out) {
+        const int at = tot_in + off_out +
+                       (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b_out >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((unsigned)b_out, 0));
+        map[1 + at] = dst_out;
+        map[1 + 2 * ONB + at] = src_out;
+    }
+    if (t == 0) map[0] = tot_in + tot_out;
 }
 
 // Applies a swap map to columns [c0, c1) and [c2, c3).  Each workgroup owns
@@ -453,20 +497,53 @@ __global__ __launch_bounds__(256) void k_laswp(T *__restrict__ A, long lda, cons
 
 // ---------------------------------------------------------------- TRSM
 
-// B := L^-1 B for the unit-lower jb x jb block L = A[j0:j0+jb, j0:j0+jb]
-// (jb <= 64) and B = A[j0:j0+jb, c0:c1), in place, column sweep:
-// for each step c, b[r] = b[r] - l[r][c] * b[c] for r > c.  This is the
-// order in which gaussian_elimination (lu.rs:603-616) updates the rows of
-// the block, with the same separately rounded multiply and subtract.
-//
-// L and the 64-column slab of B are staged in LDS with every load issued
-// before the stores; four waves split the rows of each sweep step.
+// The block-row solve B := L^-1 B of a factored panel, with L the unit-lower
+// jb x jb block A[j0:j0+jb, j0:j0+jb] (jb <= 64) and B = A[j0:j0+jb, c0:c1)
+// (the reference applies the same elimination to these columns row by row,
+// lu.rs:603-616), is done as a product with L^-1: the inverse is formed once
+// per panel, and every 64-column slab of B is then one 64-deep matrix-core
+// product instead of a 64-step dependent sweep.  L has a unit diagonal and,
+// under partial pivoting, entries of magnitude at most 1.
+
+// Row-major 64 x 64 inv = L^-1, zero outside jb x jb.  Lane j
+// forward-substitutes column j of the identity, holding the whole column in
+// registers; the rows of L come from LDS as wave-wide broadcasts.
+template <class T>
+__global__ __launch_bounds__(64) void k_tri_inv(const T *__restrict__ A, long lda, int j0, int jb,
+                                                T *__restrict__ inv) {
+    __shared__ T L[PNB][PNB];
+    const int j = threadIdx.x;
+    {
+        T v[PNB];
+#pragma unroll
+        for (int r = 0; r < PNB; ++r) v[r] = (r < jb && j < jb) ? A[(long)(j0 + r) * lda + j0 + j] : T(0);
+#pragma unroll
+        for (int r = 0; r < PNB; ++r) L[r][j] = v[r];
+    }
+    __syncthreads();
+    T x[PNB];
+#pragma unroll
+    for (int i = 0; i < PNB; ++i) {
+        T s = (i == j) ? T(1) : T(0);
+#pragma unroll
+        for (int q = 0; q < i; ++q) s -= L[i][q] * x[q];
+        x[i] = s;
+    }
+#pragma unroll
+    for (int i = 0; i < PNB; ++i) inv[i * PNB + j] = (i < jb && j < jb) ? x[i] : T(0);
+}
+
+// B := inv * B for one 64-column slab per workgroup, in place: the slab is
+// read whole into LDS before any of it is written.  Wave w computes rows
+// [16 w, 16 w + 16) of the slab as four 16 x 16 accumulators over 16 k-steps.
 template <class T>
 __global__ __launch_bounds__(256) void k_trsm_llu(T *__restrict__ A, long lda, int j0, int jb, int c0,
-                                                  int c1) {
-    __shared__ T L[PNB][PNB + 1];
-    __shared__ T X[PNB][65];
-    const int tid = threadIdx.x;
+                                                  int c1, const T *__restrict__ inv) {
+    using Acc = typename Mfma<T>::acc;
+    __shared__ T Li[PNB][PNB + 1];  // Li[k][i] = inv[i][k]
+    __shared__ T X[PNB][PNB + 1];   // X[k][c] = B[k][c]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
     const int cb = c0 + blockIdx.x * 64;
     const int nc = min(64, c1 - cb);
     {
@@ -475,41 +552,33 @@ __global__ __launch_bounds__(256) void k_trsm_llu(T *__restrict__ A, long lda, i
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = q * 256 + tid, r = e >> 6, c = e & 63;
-            vl[q] = (r < jb && c < jb) ? A[(long)(j0 + r) * lda + j0 + c] : T(0);
+            vl[q] = inv[e];
             vx[q] = (r < jb && c < nc) ? A[(long)(j0 + r) * lda + cb + c] : T(0);
         }
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = q * 256 + tid, r = e >> 6, c = e & 63;
-            L[r][c] = vl[q];
+            Li[c][r] = vl[q];
             X[r][c] = vx[q];
         }
     }
     __syncthreads();
-    // each wave owns 16 whole columns (lane = column + 16 * row group), so a
-    // column's steps only depend on LDS writes of the same wave, which the
-    // LDS applies in order: no workgroup barrier inside the sweep
-    const int cj = (tid >> 6) * 16 + (tid & 15), rq = (tid & 63) >> 4;
-    {
-#pragma clang fp contract(off)
-        for (int c = 0; c < jb - 1; ++c) {
-            const T xc = X[c][cj];
-            // the (up to 16) rows of this lane's group, unrolled so their LDS
-            // reads are issued together
+    Acc acc[4];
 #pragma unroll
-            for (int i = 0; i < PNB / 4; ++i) {
-                const int r = c + 1 + rq + 4 * i;
-                if (r < jb) X[r][cj] = X[r][cj] - L[r][c] * xc;
-            }
-            __builtin_amdgcn_wave_barrier();
-            __asm__ __volatile__("" ::: "memory");  // keep step c + 1's reads after these writes
+    for (int jc = 0; jc < 4; ++jc) acc[jc] = Acc{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < PNB / 4; ++s) {
+        const T a = Li[4 * s + lk][16 * wave + lr];
+#pragma unroll
+        for (int jc = 0; jc < 4; ++jc) acc[jc] = Mfma<T>::op(a, X[4 * s + lk][16 * jc + lr], acc[jc]);
+    }
+#pragma unroll
+    for (int jc = 0; jc < 4; ++jc)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * wave + Mfma<T>::row(lk, r), col = 16 * jc + lr;
+            if (row < jb && col < nc) A[(long)(j0 + row) * lda + cb + col] = acc[jc][r];
         }
-    }
-    __syncthreads();
-    for (int e = tid; e < jb * 64; e += 256) {
-        const int r = e >> 6, c = e & 63;
-        if (c < nc) A[(long)(j0 + r) * lda + cb + c] = X[r][c];
-    }
 }
 
 // ---------------------------------------------------------------- driver
@@ -521,32 +590,45 @@ static thread_local unsigned g_tslu_epoch = 0;
 
 template <class T>
 static int panel(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, int64_t *ipiv,
-                 int64_t *info, size_t n, const LuWork &lw, unsigned *bar);
+                 int64_t *info, size_t n, const LuWork &lw);
 
 template <class T>
 static int panel_any(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, int64_t *ipiv,
-                     int64_t *info, size_t n, const LuWork &lw, unsigned *bar) {
+                     int64_t *info, size_t n, const LuWork &lw) {
     if constexpr (std::is_same<T, float>::value) {
         if (g_tslu_work && m <= tslu_max_rows())
             return tslu_panel(s, a, lda, k, m, w, ipiv, info, n, g_tslu_work, ++g_tslu_epoch,
                               lw.timeout);
     }
-    return panel<T>(s, a, lda, k, m, w, ipiv, info, n, lw, bar);
+    return panel<T>(s, a, lda, k, m, w, ipiv, info, n, lw);
+}
+
+// Largest panel height: the panel's workgroups exchange data while they run,
+// so all of them must be resident at once (one per CU), and their slots must
+// fit the exchange area.
+template <class T>
+static int panel_rows_max(size_t *out) {
+    Device *d = nullptr;
+    RLG_TRY(current_device(&d));
+    const size_t g = (size_t)(d->num_cus < GMAX ? d->num_cus : GMAX);
+    *out = g * PanelRows<T>::value;
+    return RLG_OK;
 }
 
 template <class T>
 static int panel(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, int64_t *ipiv,
-                 int64_t *info, size_t n, const LuWork &lw, unsigned *bar) {
-    const unsigned G = ceil_div(m, PanelRows<T>::value);
-    if (G > GMAX) {
-        set_error("LU panel of %zu rows needs %u workgroups (max %d)", m, G, GMAX);
+                 int64_t *info, size_t n, const LuWork &lw) {
+    size_t mmax = 0;
+    RLG_TRY(panel_rows_max<T>(&mmax));
+    if (m > mmax) {
+        set_error("LU panel of %zu rows exceeds the %zu rows that this device's CUs hold", m, mmax);
         return RLG_E_INVALID_ARG;
     }
+    const unsigned G = ceil_div(m, PanelRows<T>::value);
     // algorithmic bytes: the m x w panel read once and written once
     RLG_PROF_W("lu_panel", s, 2.0 * (double)m * (double)w * sizeof(T));
     hipLaunchKernelGGL(k_lu_panel<T>, dim3(G), dim3(PTHREADS), 0, s, a, (long)lda, (int)k, (int)m,
-                       (int)w, ipiv, info, (int)n, bar, static_cast<T *>(lw.cval), lw.crow,
-                       static_cast<T *>(lw.cdat), static_cast<T *>(lw.rowj), lw.timeout);
+                       (int)w, ipiv, info, (int)n, lw.xch, lw.timeout);
     RLG_LAUNCHED();
     return RLG_OK;
 }
@@ -558,7 +640,11 @@ static int laswp(hipStream_t s, T *a, size_t lda, const int64_t *ipiv, size_t k0
     if (ncols == 0 || cnt == 0) return RLG_OK;
     // algorithmic bytes: at most 2 cnt rows change, each read and written
     RLG_PROF_W("laswp", s, 2.0 * 2.0 * (double)cnt * (double)ncols * sizeof(T));
-    hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(64), 0, s, ipiv, (int)k0, (int)cnt, lw.map);
+    if (cnt > (size_t)ONB) {
+        set_error("row swap block of %zu rows exceeds %d", cnt, ONB);
+        return RLG_E_INVALID_ARG;
+    }
+    hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(ONB), 0, s, ipiv, (int)k0, (int)cnt, lw.map);
     RLG_LAUNCHED();
     hipLaunchKernelGGL(k_laswp<T>, dim3(ceil_div(ncols, LCOLS)), dim3(256), 0, s, a, (long)lda,
                        lw.map, (int)c0, (int)c1, (int)c2, (int)c3);
@@ -566,13 +652,32 @@ static int laswp(hipStream_t s, T *a, size_t lda, const int64_t *ipiv, size_t k0
     return RLG_OK;
 }
 
+// The inverse of panel j0's unit-lower block: slot (j0 mod ONB) / PNB of the
+// outer block's inverses.
 template <class T>
-static int trsm_llu(hipStream_t s, T *a, size_t lda, size_t j0, size_t jb, size_t c0, size_t c1) {
+static T *linv_of(const LuWork &lw, size_t j0) {
+    return static_cast<T *>(lw.linv) + ((j0 % ONB) / PNB) * PNB * PNB;
+}
+
+template <class T>
+static int tri_inv(hipStream_t s, const T *a, size_t lda, size_t j0, size_t jb, const LuWork &lw) {
+    if (jb <= 1) return RLG_OK;
+    // algorithmic bytes: the unit-lower block read, the 64 x 64 inverse written
+    RLG_PROF_W("tri_inv", s, ((double)jb * jb / 2 + (double)PNB * PNB) * sizeof(T));
+    hipLaunchKernelGGL(k_tri_inv<T>, dim3(1), dim3(PNB), 0, s, a, (long)lda, (int)j0, (int)jb,
+                       linv_of<T>(lw, j0));
+    RLG_LAUNCHED();
+    return RLG_OK;
+}
+
+template <class T>
+static int trsm_llu(hipStream_t s, T *a, size_t lda, size_t j0, size_t jb, size_t c0, size_t c1,
+                    const LuWork &lw) {
     if (c1 <= c0 || jb <= 1) return RLG_OK;
-    // algorithmic bytes: the unit-lower jb x jb block and the jb x (c1-c0) block row (read + write)
-    RLG_PROF_W("trsm", s, ((double)jb * jb / 2 + 2.0 * (double)jb * (double)(c1 - c0)) * sizeof(T));
+    // algorithmic bytes: the 64 x 64 inverse and the jb x (c1-c0) block row (read + write)
+    RLG_PROF_W("trsm", s, ((double)PNB * PNB + 2.0 * (double)jb * (double)(c1 - c0)) * sizeof(T));
     hipLaunchKernelGGL(k_trsm_llu<T>, dim3(ceil_div(c1 - c0, 64)), dim3(256), 0, s, a, (long)lda,
-                       (int)j0, (int)jb, (int)c0, (int)c1);
+                       (int)j0, (int)jb, (int)c0, (int)c1, (const T *)linv_of<T>(lw, j0));
     RLG_LAUNCHED();
     return RLG_OK;
 }
@@ -615,11 +720,13 @@ static int outer_panel(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv,
                        const LuWork &lw, size_t k0, size_t kb) {
     for (size_t j0 = k0; j0 < k0 + kb; j0 += PNB) {
         const size_t jb = k0 + kb - j0 < (size_t)PNB ? k0 + kb - j0 : PNB;
-        RLG_TRY(panel_any<T>(s, a, lda, j0, n - j0, jb, ipiv, info, n, lw, lw.bar + (j0 / PNB) * BAR_STRIDE));
+        RLG_TRY(panel_any<T>(s, a, lda, j0, n - j0, jb, ipiv, info, n, lw));
+        // L^-1 of the panel's diagonal block, for its two block-row solves
+        if (j0 + jb < n) RLG_TRY(tri_inv<T>(s, a, lda, j0, jb, lw));
         // inner swaps on the other columns of the outer block
         RLG_TRY(laswp<T>(s, a, lda, ipiv, j0, jb, k0, j0, j0 + jb, k0 + kb, lw));
         if (j0 + jb < k0 + kb) {
-            RLG_TRY(trsm_llu<T>(s, a, lda, j0, jb, j0 + jb, k0 + kb));
+            RLG_TRY(trsm_llu<T>(s, a, lda, j0, jb, j0 + jb, k0 + kb, lw));
             RLG_TRY(update_native<T>(s, n - j0 - jb, k0 + kb - j0 - jb, jb, a + (j0 + jb) * lda + j0,
                                      lda, a + j0 * lda + j0 + jb, lda, a + (j0 + jb) * lda + j0 + jb,
                                      lda));
@@ -643,8 +750,8 @@ template <class T>
 static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv, int64_t *info,
                           void *work, UpdateFn<T> big_update) {
     LuWork lw = carve(work, n, sizeof(T));
-    const size_t panels = (n + PNB - 1) / PNB + 1;
-    RLG_TRY(fill_u32(lw.bar, panels * BAR_STRIDE, 0u, s));
+    // exchange tags restart with every factorization
+    RLG_TRY(fill_u32(reinterpret_cast<uint32_t *>(lw.xch), 2 * xch_words(sizeof(T)), 0u, s));
     RLG_TRY(fill_u32(reinterpret_cast<uint32_t *>(lw.timeout), 1, 0u, s));
     RLG_TRY(fill_i64(info, 1, (int64_t)n, s));
     if (n == 0) return RLG_OK;
@@ -668,7 +775,7 @@ static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ip
         if (c0 >= n) break;
         for (size_t j0 = k0; j0 < c0; j0 += PNB) {
             const size_t jb = c0 - j0 < (size_t)PNB ? c0 - j0 : PNB;
-            RLG_TRY(trsm_llu<T>(s, a, lda, j0, jb, c0, n));
+            RLG_TRY(trsm_llu<T>(s, a, lda, j0, jb, c0, n, lw));
             if (j0 + jb < c0)
                 RLG_TRY(update_native<T>(s, c0 - j0 - jb, n - c0, jb, a + (j0 + jb) * lda + j0, lda,
                                          a + j0 * lda + c0, lda, a + (j0 + jb) * lda + c0, lda));

@@ -231,8 +231,8 @@ def test_lu_vs_oracle(n):
 
 
 def test_lu_f32_and_mixed_wide_panels():
-    """Order 4608: the first FP32 panels span nine 512-row workgroups, so the
-    panel barrier takes its two-level form (FP64 reaches it at 2304, above)."""
+    """Order 4608: the first FP32 panels span nine 512-row workgroups that
+    exchange their pivot candidates (FP64 panels span nine from 2304, above)."""
     n = 4608
     a = wl.gaussian(n, n, 4608)
     b = wl.gaussian(1, n, 4609)[0]
