@@ -5,7 +5,7 @@
 //   C = alpha * A * op(B) + beta * C, all row-major with row strides.
 //
 // Tiling (FP64/FP32): a 256-thread workgroup owns a 128 x 128 tile of C and
-// walks K in steps of 16, double-buffered through LDS.  The four wave64s
+// walks K in steps of 16 (FP64) or 32 (FP32), double-buffered through LDS.  The four wave64s
 // are arranged 2 x 2; each owns a 64 x 64 sub-tile held as 4 x 4 MFMA
 // accumulators of 16 x 16 (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
 // A is staged k-major (As[k][i]) so that an MFMA A-fragment (lane l holds
@@ -34,8 +34,11 @@ constexpr int GBM = 128, GBN = 128, GBK = 16, GTHREADS = 256;
 __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, int &tn) {
     const int total = tiles_m * tiles_n;
     int id = blockIdx.x;
-    if (total >= 64 && (total % 8) == 0) {
-        const int per_xcd = total / 8;
+    // the first 8 * floor(total / 8) ids are regrouped; the few left over
+    // keep their own tile
+    const int full = total & ~7;
+    if (total >= 64 && id < full) {
+        const int per_xcd = full / 8;
         id = (id % 8) * per_xcd + id / 8;
     }
     constexpr int GROUP_M = 8;
@@ -48,14 +51,22 @@ __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, i
     tn = r / gm;
 }
 
+// k per LDS stage: 16 in FP64 (2 x 33 KB of stages, two workgroups per CU),
+// 32 in FP32 (the same bytes): twice the matrix-core work between barriers.
+template <class T> struct StageK { static constexpr int value = GBK; };
+template <> struct StageK<float> { static constexpr int value = 2 * GBK; };
+
 template <class T, bool TRANSB, bool LOWER>
 __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
     int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
     const T *__restrict__ B, long ldb, T beta, T *__restrict__ C, long ldc,
     int tiles_m, int tiles_n) {
     using Acc = typename Mfma<T>::acc;
-    __shared__ T As[2][GBK][GBM + 1];
-    __shared__ T Bs[2][GBK][GBN + (TRANSB ? 1 : 0)];
+    constexpr int BK = StageK<T>::value;
+    constexpr int PER = BK * GBM / GTHREADS;  // staged elements per thread per operand
+    static_assert(GBM == GBN && PER * GTHREADS == BK * GBM, "square tiles, whole stages");
+    __shared__ T As[2][BK][GBM + 1];
+    __shared__ T Bs[2][BK][GBN + (TRANSB ? 1 : 0)];
 
     int tm, tn;
     tile_coords(tiles_m, tiles_n, tm, tn);
@@ -72,22 +83,22 @@ __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
-    T ra[8], rb[8];
+    T ra[PER], rb[PER];
     auto load = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < PER; ++q) {
             const int e = q * GTHREADS + tid;
-            {   // A tile, 128 x 16: e -> (i = e / 16, kk = e % 16)
-                const int i = e >> 4, kk = e & 15;
+            {   // A tile, 128 x BK: e -> (i = e / BK, kk = e % BK)
+                const int i = e / BK, kk = e % BK;
                 const int gi = m0 + i, gk = k0 + kk;
                 ra[q] = (gi < M && gk < K) ? A[(long)gi * lda + gk] : T(0);
             }
-            if (!TRANSB) {  // B tile, 16 x 128: e -> (kk = e / 128, j = e % 128)
-                const int kk = e >> 7, j = e & 127;
+            if (!TRANSB) {  // B tile, BK x 128: e -> (kk = e / 128, j = e % 128)
+                const int kk = e / GBN, j = e % GBN;
                 const int gk = k0 + kk, gj = n0 + j;
                 rb[q] = (gk < K && gj < N) ? B[(long)gk * ldb + gj] : T(0);
-            } else {        // B^T given as N x K: e -> (j = e / 16, kk = e % 16)
-                const int j = e >> 4, kk = e & 15;
+            } else {        // B^T given as N x K: e -> (j = e / BK, kk = e % BK)
+                const int j = e / BK, kk = e % BK;
                 const int gj = n0 + j, gk = k0 + kk;
                 rb[q] = (gj < N && gk < K) ? B[(long)gj * ldb + gk] : T(0);
             }
@@ -95,25 +106,25 @@ __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < PER; ++q) {
             const int e = q * GTHREADS + tid;
-            As[buf][e & 15][e >> 4] = ra[q];
+            As[buf][e % BK][e / BK] = ra[q];
             if (!TRANSB)
-                Bs[buf][e >> 7][e & 127] = rb[q];
+                Bs[buf][e / GBN][e % GBN] = rb[q];
             else
-                Bs[buf][e & 15][e >> 4] = rb[q];
+                Bs[buf][e % BK][e / BK] = rb[q];
         }
     };
 
-    const int KT = (K + GBK - 1) / GBK;
+    const int KT = (K + BK - 1) / BK;
     load(0);
     store(0);
     __syncthreads();
     for (int kt = 0; kt < KT; ++kt) {
         const int buf = kt & 1;
-        if (kt + 1 < KT) load((kt + 1) * GBK);
+        if (kt + 1 < KT) load((kt + 1) * BK);
 #pragma unroll
-        for (int ks = 0; ks < GBK / 4; ++ks) {
+        for (int ks = 0; ks < BK / 4; ++ks) {
             T af[4], bf[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) af[i] = As[buf][ks * 4 + lk][wm + i * 16 + lr];

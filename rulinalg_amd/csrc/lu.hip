@@ -107,11 +107,8 @@ void perm_from_ipiv(size_t n, const int64_t *ipiv, int64_t *raw) {
 // row index, which reproduces the reference's first-maximum scan.
 template <class T>
 __device__ __forceinline__ bool better(T v2, int i2, T v1, int i1) {
-    if (i2 == INT_MAX) return false;
-    if (i1 == INT_MAX) return true;
-    if (v2 > v1) return true;
-    if (v1 > v2) return false;
-    return i2 < i1;
+    // branch-free: the exec-mask branches of an if-chain cost more than the tests
+    return (i2 != INT_MAX) & ((i1 == INT_MAX) | (v2 > v1) | (!(v1 > v2) & (i2 < i1)));
 }
 
 // (v, i) and a payload := the wave's best under `better`, in every lane.
@@ -171,43 +168,64 @@ __device__ __forceinline__ bool xwait(unsigned &spins, int *timeout) {
 // Factor the panel A[k:k+m, k:k+w] (w <= PNB) in place.
 //
 // Workgroup g holds rows [g PRT, (g+1) PRT) of the panel in LDS; thread t
-// owns the tile rows t + h PTHREADS and does every local step on them, so a
-// column needs only the two workgroup barriers around wave 0's exchange.
-// Column j:
-//   1. every thread finds the first maximum |a[r][j]| over its rows r >= j;
-//   2. wave 0 reduces the wave candidates to the workgroup's, publishes it
-//      (row index, |value| and the whole row) and, if it holds row j, row j;
-//      reads all G candidates and picks the same winner as every other
-//      workgroup; fetches the winning row unless it is its own and, if it
-//      holds the winning row but not row j, row j; swaps the rows it holds
-//      and puts the pivot row in LDS;
-//   3. every thread eliminates its rows below j.
+// owns the tile rows t + h PTHREADS and does every local step on them.  The
+// elimination is lazy by one column: pivot j is applied to column j + 1 at
+// once, and to the columns beyond during the next column's exchange, while
+// the workgroup waits for the other candidates anyway.  Column j:
+//   A. every thread finds the first maximum |a[r][j]| over its rows r >= j
+//      (column j is up to date); each wave copies its candidate row, and the
+//      wave holding row j that row, to LDS; barrier;
+//   B. wave 0 picks the workgroup's candidate and brings the copied rows up
+//      to date with pivot j - 1 (the arithmetic the owners apply in place),
+//      then publishes the candidate (row index, |value|, the row) and, if it
+//      holds row j, row j;
+//   C. every thread applies pivot j - 1 to columns > j of its rows; barrier;
+//   D. wave 0 reads all G candidates and picks the same winner as every
+//      other workgroup, fetches the winning row unless it is its own and, if
+//      it holds the winning row but not row j, row j; swaps the rows it
+//      holds and puts the pivot row in LDS; barrier;
+//   E. every thread turns column j of its rows below j into multipliers and
+//      applies pivot j to column j + 1.
 // The pivot indices stay in LDS until the end: a global store before a
 // workgroup barrier would make the barrier wait for its completion.
 
 // Tile rows are padded to a 16-byte multiple: 16-byte aligned rows let the
-// row sweeps use 16-byte LDS accesses, and the odd multiple of 16 bytes
-// spreads a column over the banks.
+// row sweeps use 16-byte LDS accesses.
 template <class T> struct TilePad { static constexpr int value = 16 / sizeof(T); };
 template <class T> constexpr int TW = PNB + TilePad<T>::value;
 
-// a[r][c] = a[r][c] - mult * prow[c] for c > j, as one straight sweep over
-// the columns from C0 on (C0 <= j + 1): every column is read, and written
-// back unchanged where c <= j, so that no column costs a branch (a branch
-// per column serializes one LDS round trip per column) and the reads go out
-// back to back.  Columns w..63 of the tile and of prow hold zeros and stay
-// zero.  The caller writes column j afterwards.
+// a[r][c] = a[r][c] - mult * pm[c] over the columns from C0 on, as one
+// straight sweep.  pm is the pivot row with zeros in the columns that must
+// stay as they are (x - mult * 0 = x, up to the sign of a zero x), so every
+// column is read and written without a branch or select: a per-column test
+// gets compiled into a conditional store, one LDS round trip per column.
+// Columns w..63 of the tile and of pm hold zeros and stay zero.
 template <class T, int C0>
-__device__ __forceinline__ void elim_row(T (*__restrict__ tile)[TW<T>], const T *__restrict__ prow,
-                                         int r, int j, T mult) {
+__device__ __forceinline__ void elim_row(T (*__restrict__ tile)[TW<T>], const T *__restrict__ pm,
+                                         int r, T mult) {
 #pragma clang fp contract(off)
     T x[PNB - C0];
 #pragma unroll
     for (int c = C0; c < PNB; ++c) x[c - C0] = tile[r][c];
 #pragma unroll
-    for (int c = C0; c < PNB; ++c) {
-        const T y = x[c - C0] - mult * prow[c];
-        tile[r][c] = c > j ? y : x[c - C0];
+    for (int c = C0; c < PNB; ++c) tile[r][c] = x[c - C0] - mult * pm[c];
+}
+
+// elim_row from the 8-column chunk that holds column j + 1 (pm is zero at
+// columns <= j + 1 for the pending update of column j's step).
+template <class T>
+__device__ __forceinline__ void elim_row_from(T (*__restrict__ tile)[TW<T>], const T *__restrict__ pm,
+                                              int r, int j, T mult) {
+    switch ((j + 1) >> 3) {
+        case 0: elim_row<T, 0>(tile, pm, r, mult); break;
+        case 1: elim_row<T, 8>(tile, pm, r, mult); break;
+        case 2: elim_row<T, 16>(tile, pm, r, mult); break;
+        case 3: elim_row<T, 24>(tile, pm, r, mult); break;
+        case 4: elim_row<T, 32>(tile, pm, r, mult); break;
+        case 5: elim_row<T, 40>(tile, pm, r, mult); break;
+        case 6: elim_row<T, 48>(tile, pm, r, mult); break;
+        case 7: elim_row<T, 56>(tile, pm, r, mult); break;
+        default: break;  // j = 63: nothing right of column j
     }
 }
 
@@ -219,11 +237,17 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
     constexpr int PRT = PanelRows<T>::value;
     constexpr int RPT = PRT / PTHREADS;  // panel rows per thread
     static_assert(RPT * PTHREADS == PRT, "whole rows per thread");
+    constexpr int NW = PTHREADS / 64;
     constexpr int VW = XSlot<T>::VW, XS = XSlot<T>::words;
     __shared__ __attribute__((aligned(16))) T tile[PRT][TW<T>];
-    __shared__ __attribute__((aligned(16))) T prow[PNB];
-    __shared__ T red_v[PTHREADS / 64];
-    __shared__ int red_i[PTHREADS / 64];
+    __shared__ __attribute__((aligned(16))) T prow[PNB];      // pivot row j
+    // pivot row j with zeros at columns <= j + 1, by column parity (its
+    // pending update is applied during column j + 1's step)
+    __shared__ __attribute__((aligned(16))) T pmask[2][PNB];
+    __shared__ T wrow[NW][PNB];                                // wave candidates' rows
+    __shared__ T jrow[PNB];                                    // row j
+    __shared__ T red_v[NW];
+    __shared__ int red_i[NW];
     __shared__ int s_piv[PNB];
 
     const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
@@ -250,35 +274,49 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
     bool live = true;  // wave 0: false once an exchange has timed out
     int small = w;     // wave 0: first column whose pivot is below epsilon
     for (int j = 0; j < w; ++j) {
-        // 1. this thread's candidate over its rows >= j
+        const T *pmprev = pmask[(j + 1) & 1];  // masked pivot row j - 1
+        const bool own_j = j >= r0 && j < r0 + rows;
+
+        // A. this thread's candidate, the wave's, and the rows wave 0 needs
         T bv = T(0);
         int bi = INT_MAX;
 #pragma unroll
         for (int h = 0; h < RPT; ++h) {
             const int r = tid + h * PTHREADS;
-            if (r < rows && r0 + r >= j) {
-                const T v = fabs(tile[r][j]);
-                if (better(v, r0 + r, bv, bi)) { bv = v; bi = r0 + r; }
-            }
+            const T v = fabs(tile[r][j]);
+            const bool take = (r < rows) & (r0 + r >= j) & better(v, r0 + r, bv, bi);
+            bv = take ? v : bv;
+            bi = take ? r0 + r : bi;
         }
         {
             int unused = 0;
             wave_argmax(bv, bi, unused);
         }
+        if (bi != INT_MAX) wrow[wave][lane] = tile[bi - r0][lane];
+        if (own_j && (((j - r0) % PTHREADS) >> 6) == wave) jrow[lane] = tile[j - r0][lane];
         if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
         __syncthreads();
 
+        // B. the workgroup's candidate row and row j, up to date, published
+        T pv = T(0), oj = T(0);
         if (wave == 0) {
-            // 2. the workgroup's candidate (uniform over the wave)
+#pragma clang fp contract(off)
+            int bw = 0;
 #pragma unroll
-            for (int q = 1; q < PTHREADS / 64; ++q)
-                if (better(red_v[q], red_i[q], bv, bi)) { bv = red_v[q]; bi = red_i[q]; }
-            const bool own_j = j >= r0 && j < r0 + rows;
-            // lane c: column c of the candidate row and of row j
-            T pv = (lane < w && bi != INT_MAX) ? tile[bi - r0][lane] : T(0);
-            T oj = (lane < w && own_j) ? tile[j - r0][lane] : T(0);
-            int p = bi;
-            T pabs = bv;
+            for (int q = 1; q < NW; ++q) {
+                const bool take = better(red_v[q], red_i[q], bv, bi);
+                bv = take ? red_v[q] : bv;
+                bi = take ? red_i[q] : bi;
+                bw = take ? q : bw;
+            }
+            if (bi != INT_MAX) {
+                const T x = wrow[bw][lane];
+                pv = j > 0 ? x - wrow[bw][j - 1] * pmprev[lane] : x;
+            }
+            if (own_j) {
+                const T x = jrow[lane];
+                oj = j > 0 ? x - jrow[j - 1] * pmprev[lane] : x;
+            }
             if (G > 1 && live) {
                 const unsigned tag = (unsigned)(k + j + 1);
                 xword *base = xch + (size_t)(j & 1) * XPAR * XS;
@@ -291,6 +329,27 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
                     xput(mine, tag, (unsigned)bi);
                     xput_val(mine + 1, tag, bv);
                 }
+            }
+        }
+
+        // C. pivot j - 1 on the columns right of j
+        if (j > 0) {
+#pragma unroll
+            for (int h = 0; h < RPT; ++h) {
+                const int r = tid + h * PTHREADS;
+                if (r < rows && r0 + r >= j) elim_row_from(tile, pmprev, r, j, tile[r][j - 1]);
+            }
+        }
+        __syncthreads();
+
+        // D. the winner, the swap and the pivot row
+        if (wave == 0) {
+            int p = bi;
+            T pabs = bv;
+            if (G > 1 && live) {
+                const unsigned tag = (unsigned)(k + j + 1);
+                const xword *base = xch + (size_t)(j & 1) * XPAR * XS;
+                const xword *diag = base + (size_t)GMAX * XS;
                 // every workgroup's candidate, 64 slots per round
                 T v = T(0);
                 int i = INT_MAX, gw = 0;
@@ -310,7 +369,10 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
                         if (__all(ok)) break;
                         if (!xwait(spins, timeout)) { live = false; break; }
                     }
-                    if (q < G && better(qv, qi, v, i)) { v = qv; i = qi; gw = q; }
+                    const bool take = (q < G) & better(qv, qi, v, i);
+                    v = take ? qv : v;
+                    i = take ? qi : i;
+                    gw = take ? q : gw;
                 }
                 wave_argmax(v, i, gw);
                 if (live) {
@@ -346,33 +408,24 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
                 if (p >= r0 && p < r0 + rows) tile[p - r0][lane] = oj;
             }
             prow[lane] = pv;
+            pmask[j & 1][lane] = lane > j + 1 ? pv : T(0);
             if (lane == 0) s_piv[j] = p;
             if (pabs < eps_of<T>() && small == w) small = j;
         }
         __syncthreads();
 
-        // 3. eliminate: mult = a[i][j] / piv; a[i][c] = a[i][c] - mult * piv_row[c],
-        //    swept from the 8-column chunk that holds column j + 1
+        // E. multipliers of column j; pivot j on column j + 1
         {
 #pragma clang fp contract(off)
             const T piv = prow[j];
+            const T pn = j + 1 < PNB ? prow[j + 1] : T(0);
 #pragma unroll
             for (int h = 0; h < RPT; ++h) {
                 const int r = tid + h * PTHREADS;
                 if (r < rows && r0 + r > j) {
                     const T mult = tile[r][j] / piv;
-                    switch ((j + 1) >> 3) {
-                        case 0: elim_row<T, 0>(tile, prow, r, j, mult); break;
-                        case 1: elim_row<T, 8>(tile, prow, r, j, mult); break;
-                        case 2: elim_row<T, 16>(tile, prow, r, j, mult); break;
-                        case 3: elim_row<T, 24>(tile, prow, r, j, mult); break;
-                        case 4: elim_row<T, 32>(tile, prow, r, j, mult); break;
-                        case 5: elim_row<T, 40>(tile, prow, r, j, mult); break;
-                        case 6: elim_row<T, 48>(tile, prow, r, j, mult); break;
-                        case 7: elim_row<T, 56>(tile, prow, r, j, mult); break;
-                        default: break;  // j = 63: nothing right of the pivot
-                    }
                     tile[r][j] = mult;
+                    if (j + 1 < PNB) tile[r][j + 1] = tile[r][j + 1] - mult * pn;
                 }
             }
         }
