@@ -111,10 +111,33 @@ __device__ __forceinline__ bool better(T v2, int i2, T v1, int i1) {
     return (i2 != INT_MAX) & ((i1 == INT_MAX) | (v2 > v1) | (!(v1 > v2) & (i2 < i1)));
 }
 
-// (v, i) and a payload := the wave's best under `better`, in every lane.
-template <class T>
-__device__ __forceinline__ void wave_argmax(T &v, int &i, int &pay) {
-    wave_argmax_dpp(v, i, pay, [](T v2, int i2, T v1, int i1) { return better(v2, i2, v1, i1); });
+// The lane that holds the wave's first maximum under `better` (largest
+// |value|, ties to the smallest row index; i == INT_MAX marks a lane without
+// a candidate), or -1 if no lane has one; uniform.  One 64-bit maximum over
+// the wave: in FP32 the key packs the value's bits (a non-negative float
+// orders like its bits) above the complemented row index, so keys are
+// distinct; in FP64 the key is the value's bits + 1 and a tie is settled by
+// the smallest index among the tied lanes.  A NaN ranks like zero.
+__device__ __forceinline__ int wave_first_max_lane(float v, int i) {
+    unsigned long long key = 0;
+    if (i != INT_MAX) {
+        const unsigned b = (v == v) ? __float_as_uint(v) : 0u;
+        key = ((unsigned long long)b << 32) | (0xFFFFFFFFu - (unsigned)i);
+    }
+    const unsigned long long m = wave_max_u64(key);
+    if (m == 0) return -1;
+    return __ffsll((long long)__ballot(key == m)) - 1;
+}
+__device__ __forceinline__ int wave_first_max_lane(double v, int i) {
+    unsigned long long key = 0;
+    if (i != INT_MAX) key = (v == v) ? (unsigned long long)__double_as_longlong(v) + 1 : 1;
+    const unsigned long long m = wave_max_u64(key);
+    if (m == 0) return -1;
+    const unsigned long long tied = __ballot(key == m);
+    if (__popcll(tied) == 1) return __ffsll((long long)tied) - 1;
+    const int ii = key == m ? i : INT_MAX;
+    const int mi = wave_min_i32(ii);
+    return __ffsll((long long)__ballot(ii == mi)) - 1;
 }
 
 __device__ __forceinline__ unsigned xtag(xword w) { return (unsigned)(w >> 32); }
@@ -289,18 +312,28 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
             bi = take ? r0 + r : bi;
         }
         {
-            int unused = 0;
-            wave_argmax(bv, bi, unused);
+            const int wl = wave_first_max_lane(bv, bi);
+            bv = wl >= 0 ? lane_get(bv, wl) : T(0);
+            bi = wl >= 0 ? lane_get(bi, wl) : INT_MAX;
         }
-        if (bi != INT_MAX) wrow[wave][lane] = tile[bi - r0][lane];
-        if (own_j && (((j - r0) % PTHREADS) >> 6) == wave) jrow[lane] = tile[j - r0][lane];
+        {   // the wave's candidate row and row j, brought up to date with pivot
+            // j - 1 (the arithmetic step C applies to the rows in place)
+#pragma clang fp contract(off)
+            if (bi != INT_MAX) {
+                const T x = tile[bi - r0][lane];
+                wrow[wave][lane] = j > 0 ? x - tile[bi - r0][j - 1] * pmprev[lane] : x;
+            }
+            if (own_j && (((j - r0) % PTHREADS) >> 6) == wave) {
+                const T x = tile[j - r0][lane];
+                jrow[lane] = j > 0 ? x - tile[j - r0][j - 1] * pmprev[lane] : x;
+            }
+        }
         if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
         __syncthreads();
 
-        // B. the workgroup's candidate row and row j, up to date, published
+        // B. the workgroup's candidate row and row j, published
         T pv = T(0), oj = T(0);
         if (wave == 0) {
-#pragma clang fp contract(off)
             int bw = 0;
 #pragma unroll
             for (int q = 1; q < NW; ++q) {
@@ -309,14 +342,8 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
                 bi = take ? red_i[q] : bi;
                 bw = take ? q : bw;
             }
-            if (bi != INT_MAX) {
-                const T x = wrow[bw][lane];
-                pv = j > 0 ? x - wrow[bw][j - 1] * pmprev[lane] : x;
-            }
-            if (own_j) {
-                const T x = jrow[lane];
-                oj = j > 0 ? x - jrow[j - 1] * pmprev[lane] : x;
-            }
+            if (bi != INT_MAX) pv = wrow[bw][lane];
+            if (own_j) oj = jrow[lane];
             if (G > 1 && live) {
                 const unsigned tag = (unsigned)(k + j + 1);
                 xword *base = xch + (size_t)(j & 1) * XPAR * XS;
@@ -374,7 +401,18 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
                     i = take ? qi : i;
                     gw = take ? q : gw;
                 }
-                wave_argmax(v, i, gw);
+                {
+                    const int wl = wave_first_max_lane(v, i);
+                    if (wl >= 0) {
+                        v = lane_get(v, wl);
+                        i = lane_get(i, wl);
+                        gw = lane_get(gw, wl);
+                    } else if (live) {  // no candidate anywhere: cannot happen; report it
+                        if (lane == 0)
+                            __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        live = false;
+                    }
+                }
                 if (live) {
                     p = i;
                     pabs = v;

@@ -1,7 +1,6 @@
-// Wave-level helpers for gfx950 (wave64): lane broadcasts and a first-max
-// (argmax) reduction over the wave built from DPP moves, which cost a few
-// cycles each, where a shuffle through the LDS crossbar (ds_bpermute) costs
-// about a hundred.
+// Wave-level helpers for gfx950 (wave64): lane broadcasts and reductions over
+// the wave built from DPP moves, which cost a few cycles each, where a
+// shuffle through the LDS crossbar (ds_bpermute) costs about a hundred.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -40,34 +39,34 @@ template <int CTRL> __device__ __forceinline__ double dpp(double x) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Argmax over the wave of (v, i) under the strict order `Better`, carrying a
-// payload; every lane ends with the winner.  The order is total, so any
-// pairing of lanes gives the same result: two quad swaps and two mirrors
-// leave each row of 16 lanes with its best, and the four rows are combined
-// from lanes 0, 16, 32 and 48.
-template <class T, class Better>
-__device__ __forceinline__ void wave_argmax_dpp(T &v, int &i, int &pay, Better better) {
-    auto step = [&](auto ctrl) {
-        constexpr int C = decltype(ctrl)::value;
-        const T ov = dpp<C>(v);
-        const int oi = dpp<C>(i), op = dpp<C>(pay);
-        if (better(ov, oi, v, i)) { v = ov; i = oi; pay = op; }
-    };
-    step(std::integral_constant<int, DPP_XOR1>{});
-    step(std::integral_constant<int, DPP_XOR2>{});
-    step(std::integral_constant<int, DPP_HALF_MIRROR>{});
-    step(std::integral_constant<int, DPP_MIRROR>{});
-    T bv = lane_get(v, 0);
-    int bi = lane_get(i, 0), bp = lane_get(pay, 0);
-#pragma unroll
-    for (int q = 1; q < 4; ++q) {
-        const T qv = lane_get(v, 16 * q);
-        const int qi = lane_get(i, 16 * q), qp = lane_get(pay, 16 * q);
-        if (better(qv, qi, bv, bi)) { bv = qv; bi = qi; bp = qp; }
-    }
-    v = bv;
-    i = bi;
-    pay = bp;
+__device__ __forceinline__ unsigned long long lane_get(unsigned long long x, int l) {
+    const int lo = __builtin_amdgcn_readlane((int)x, l);
+    const int hi = __builtin_amdgcn_readlane((int)(x >> 32), l);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+template <int CTRL> __device__ __forceinline__ unsigned long long dpp(unsigned long long x) {
+    const int lo = dpp<CTRL>((int)x), hi = dpp<CTRL>((int)(x >> 32));
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
+// Maximum of a 64-bit key over the wave (uniform result): two quad swaps
+// and two row mirrors, then the four rows' maxima from lanes 0, 16, 32, 48.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k) {
+    auto mx = [](unsigned long long a, unsigned long long b) { return a > b ? a : b; };
+    k = mx(k, dpp<DPP_XOR1>(k));
+    k = mx(k, dpp<DPP_XOR2>(k));
+    k = mx(k, dpp<DPP_HALF_MIRROR>(k));
+    k = mx(k, dpp<DPP_MIRROR>(k));
+    return mx(mx(lane_get(k, 0), lane_get(k, 16)), mx(lane_get(k, 32), lane_get(k, 48)));
+}
+
+// Minimum of an int over the wave (uniform result), the same pattern.
+__device__ __forceinline__ int wave_min_i32(int k) {
+    k = min(k, dpp<DPP_XOR1>(k));
+    k = min(k, dpp<DPP_XOR2>(k));
+    k = min(k, dpp<DPP_HALF_MIRROR>(k));
+    k = min(k, dpp<DPP_MIRROR>(k));
+    return min(min(lane_get(k, 0), lane_get(k, 16)), min(lane_get(k, 32), lane_get(k, 48)));
 }
 
 }  // namespace rlg
