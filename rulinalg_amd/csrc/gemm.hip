@@ -167,131 +167,6 @@ __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
     }
 }
 
-// The same product with both operands staged k-contiguous (As[i][k],
-// Bs[j][k]) and the k index of each MFMA lane permuted: within a stage of BK,
-// lane (lr, lk) takes k = lk * (BK / 4) + s at k-step s.  An MFMA sums over
-// whatever four k its lanes supply as long as A and B use the same ones, so
-// the permutation leaves the product unchanged, and each lane's BK / 4
-// values of a fragment are contiguous: two 16-byte LDS reads per fragment
-// per stage instead of BK / 4 single reads.  Rows are padded to 144 bytes,
-// which puts the 16 rows of a fragment read on 16 distinct 16-byte bank
-// groups.
-template <class T> constexpr int KC_PAD = 16 / sizeof(T);
-
-template <class T, bool TRANSB, bool LOWER>
-__global__ __launch_bounds__(GTHREADS, 2) void k_gemm_kc(
-    int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
-    const T *__restrict__ B, long ldb, T beta, T *__restrict__ C, long ldc,
-    int tiles_m, int tiles_n) {
-    using Acc = typename Mfma<T>::acc;
-    constexpr int BK = StageK<T>::value;     // 32 (FP32) or 16 (FP64)
-    constexpr int KL = BK / 4;               // k per lane per stage
-    constexpr int S = BK + KC_PAD<T>;        // row stride in elements (144 bytes)
-    constexpr int PER = BK * GBM / GTHREADS;
-    static_assert(S * sizeof(T) == 144 && KL * sizeof(T) == 32, "two 16-byte reads per fragment");
-    __shared__ __attribute__((aligned(16))) T As[2][GBM][S];
-    __shared__ __attribute__((aligned(16))) T Bs[2][GBN][S];
-
-    int tm, tn;
-    tile_coords(tiles_m, tiles_n, tm, tn);
-    if (LOWER && tn > tm) return;  // strictly above the diagonal
-    const int m0 = tm * GBM, n0 = tn * GBN;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-    const int lr = lane & 15, lk = lane >> 4;
-
-    Acc acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
-
-    T ra[PER], rb[PER];
-    auto load = [&](int k0) {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int e = q * GTHREADS + tid;
-            {   // A tile, 128 x BK: e -> (i = e / BK, kk = e % BK)
-                const int i = e / BK, kk = e % BK;
-                const int gi = m0 + i, gk = k0 + kk;
-                ra[q] = (gi < M && gk < K) ? A[(long)gi * lda + gk] : T(0);
-            }
-            if (!TRANSB) {  // B tile, BK x 128: e -> (kk = e / 128, j = e % 128)
-                const int kk = e / GBN, j = e % GBN;
-                const int gk = k0 + kk, gj = n0 + j;
-                rb[q] = (gk < K && gj < N) ? B[(long)gk * ldb + gj] : T(0);
-            } else {        // B^T given as N x K: e -> (j = e / BK, kk = e % BK)
-                const int j = e / BK, kk = e % BK;
-                const int gj = n0 + j, gk = k0 + kk;
-                rb[q] = (gj < N && gk < K) ? B[(long)gj * ldb + gk] : T(0);
-            }
-        }
-    };
-    auto store = [&](int buf) {
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int e = q * GTHREADS + tid;
-            As[buf][e / BK][e % BK] = ra[q];
-            if (!TRANSB)
-                Bs[buf][e % GBN][e / GBN] = rb[q];
-            else
-                Bs[buf][e / BK][e % BK] = rb[q];
-        }
-    };
-
-    const int KT = (K + BK - 1) / BK;
-    load(0);
-    store(0);
-    __syncthreads();
-    for (int kt = 0; kt < KT; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < KT) load((kt + 1) * BK);
-        T af[4][KL], bf[4][KL];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int s = 0; s < KL; ++s) af[i][s] = As[buf][wm + i * 16 + lr][lk * KL + s];
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int s = 0; s < KL; ++s) bf[j][s] = Bs[buf][wn + j * 16 + lr][lk * KL + s];
-#pragma unroll
-        for (int s = 0; s < KL; ++s)
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(af[i][s], bf[j][s], acc[i][j]);
-        if (kt + 1 < KT) store(buf ^ 1);
-        __syncthreads();
-    }
-
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        T cin[4][4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gi = m0 + wm + i * 16 + Mfma<T>::row(lk, r);
-                const int gj = n0 + wn + j * 16 + lr;
-                cin[j][r] = (beta != T(0) && gi < M && gj < N) ? C[(long)gi * ldc + gj] : T(0);
-            }
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gi = m0 + wm + i * 16 + Mfma<T>::row(lk, r);
-                const int gj = n0 + wn + j * 16 + lr;
-                if (gi < M && gj < N) {
-                    T v = alpha * acc[i][j][r];
-                    if (beta != T(0)) v += beta * cin[j][r];
-                    C[(long)gi * ldc + gj] = v;
-                }
-            }
-    }
-}
-
 // ---------------------------------------------------------------- low precision
 
 template <class H> struct LowpMfma;

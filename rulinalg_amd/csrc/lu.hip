@@ -61,11 +61,10 @@ template <class T> struct XSlot {
 constexpr int XPAR = GMAX + 1;
 
 // Workspace layout (bytes): exchange slots, inverses of the unit-lower
-// diagonal blocks of the current outer block, swap map, timeout flag.
+// diagonal blocks of the current outer block, timeout flag.
 struct LuWork {
     xword *xch;        // [2][XPAR][XSlot<T>::words]
     void *linv;        // [ONB / PNB][PNB][PNB] T
-    int *map;          // [1 + 2 * 2 * ONB]: count, dst[], src[]
     int *timeout;      // exchange timeout flag
 };
 
@@ -77,7 +76,7 @@ static size_t xch_words(size_t elem) {
 size_t lu_work_bytes(size_t n, size_t elem) {
     (void)n;
     return align_up(xch_words(elem) * sizeof(xword)) + align_up((size_t)ONB * PNB * elem) +
-           align_up((1 + 4 * ONB) * 4) + align_up(4);
+           align_up(4);
 }
 
 static LuWork carve(void *work, size_t n, size_t elem) {
@@ -86,7 +85,6 @@ static LuWork carve(void *work, size_t n, size_t elem) {
     LuWork w;
     w.xch = reinterpret_cast<xword *>(p); p += align_up(xch_words(elem) * sizeof(xword));
     w.linv = p; p += align_up((size_t)ONB * PNB * elem);
-    w.map = reinterpret_cast<int *>(p); p += align_up((1 + 4 * ONB) * 4);
     w.timeout = reinterpret_cast<int *>(p);
     return w;
 }
@@ -501,10 +499,10 @@ __device__ __forceinline__ int swap_chase(const int *P, int k0, int x, int t) {
     return x;
 }
 
-__global__ __launch_bounds__(ONB) void k_swap_map(const int64_t *__restrict__ ipiv, int k0, int cnt,
-                                                  int *__restrict__ map) {
-    __shared__ int P[ONB];
-    __shared__ int wcount[2][ONB / 64];
+// The list into LDS (dst[], src[]) by the 256 threads of a workgroup, in
+// the order above; returns its length (at most 2 cnt).  P is scratch.
+__device__ int swap_map_lds(const int64_t *__restrict__ ipiv, int k0, int cnt, int *P, int *dst,
+                            int *src, int (*wcount)[ONB / 64]) {
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     if (t < cnt) P[t] = (int)ipiv[k0 + t];
     __syncthreads();
@@ -533,41 +531,44 @@ __global__ __launch_bounds__(ONB) void k_swap_map(const int64_t *__restrict__ ip
     if (mv_in) {
         const int at = off_in + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b_in >> 32),
                                                                __builtin_amdgcn_mbcnt_lo((unsigned)b_in, 0));
-        map[1 + at] = k0 + t;
-        map[1 + 2 * ONB + at] = src_in;
+        dst[at] = k0 + t;
+        src[at] = src_in;
     }
     if (mv_out) {
         const int at = tot_in + off_out +
                        (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b_out >> 32),
                                                       __builtin_amdgcn_mbcnt_lo((unsigned)b_out, 0));
-        map[1 + at] = dst_out;
-        map[1 + 2 * ONB + at] = src_out;
+        dst[at] = dst_out;
+        src[at] = src_out;
     }
-    if (t == 0) map[0] = tot_in + tot_out;
+    __syncthreads();
+    return tot_in + tot_out;
 }
 
-// Applies a swap map to columns [c0, c1) and [c2, c3).  Each workgroup owns
+// Applies the swaps ipiv[k0 .. k0+cnt) (cnt <= ONB) to columns [c0, c1) and
+// [c2, c3).  Every workgroup derives the row map itself (a few microseconds
+// of its 256 threads, where a separate launch would cost more), then owns
 // LCOLS columns: it reads every source row of the map into LDS, then writes
 // the destinations, so the in-place permutation needs no ordering.
 constexpr int LCOLS = 32;
 template <class T>
-__global__ __launch_bounds__(256) void k_laswp(T *__restrict__ A, long lda, const int *__restrict__ map,
-                                               int c0, int c1, int c2, int c3) {
+__global__ __launch_bounds__(ONB) void k_laswp(T *__restrict__ A, long lda, const int64_t *__restrict__ ipiv,
+                                               int k0, int cnt_sw, int c0, int c1, int c2, int c3) {
     __shared__ T stage[2 * ONB][LCOLS];
     __shared__ int dst[2 * ONB], src[2 * ONB];
-    const int cnt = map[0];
+    __shared__ int P[ONB];
+    __shared__ int wcount[2][ONB / 64];
+    const int cnt = swap_map_lds(ipiv, k0, cnt_sw, P, dst, src, wcount);
     if (cnt == 0) return;
-    for (int i = threadIdx.x; i < cnt; i += 256) { dst[i] = map[1 + i]; src[i] = map[1 + 2 * ONB + i]; }
     // column block -> absolute columns
     const int w1 = c1 - c0;
     const int cb = blockIdx.x * LCOLS;
-    __syncthreads();
     const int cl = threadIdx.x % LCOLS, rg = threadIdx.x / LCOLS;  // 8 row groups
     const int lc = cb + cl;
     const int col = lc < w1 ? c0 + lc : c2 + (lc - w1);
     const bool okc = lc < w1 + (c3 - c2);
     // eight loads in flight per thread before their LDS stores
-    constexpr int RG = 256 / LCOLS;
+    constexpr int RG = ONB / LCOLS;
     for (int i0 = rg; i0 < cnt; i0 += 8 * RG) {
         T v[8];
 #pragma unroll
@@ -582,7 +583,7 @@ __global__ __launch_bounds__(256) void k_laswp(T *__restrict__ A, long lda, cons
         }
     }
     __syncthreads();
-    for (int i = rg; i < cnt; i += 256 / LCOLS)
+    for (int i = rg; i < cnt; i += RG)
         if (okc) A[(long)dst[i] * lda + col] = stage[i][cl];
 }
 
@@ -596,32 +597,80 @@ __global__ __launch_bounds__(256) void k_laswp(T *__restrict__ A, long lda, cons
 // product instead of a 64-step dependent sweep.  L has a unit diagonal and,
 // under partial pivoting, entries of magnitude at most 1.
 
-// Row-major 64 x 64 inv = L^-1, zero outside jb x jb.  Lane j
-// forward-substitutes column j of the identity, holding the whole column in
-// registers; the rows of L come from LDS as wave-wide broadcasts.
+// Row-major 64 x 64 inv = L^-1, zero outside jb x jb, by halves: with L = [L11 0; L21 L22] in 32 x 32 blocks,
+// L^-1 = [X11 0; -X22 L21 X11, X22], X11 = L11^-1 and X22 = L22^-1.  Waves 0
+// and 1 invert the two diagonal halves side by side, lane j forward-
+// substituting column j of the identity with the column in registers and L
+// from LDS as broadcasts; the corner is then two 32 x 32 matrix-core
+// products, one 16 x 16 tile per wave.  The rows and columns past jb stage
+// as zeros, so they invert to the identity and are masked on the way out.
 template <class T>
-__global__ __launch_bounds__(64) void k_tri_inv(const T *__restrict__ A, long lda, int j0, int jb,
-                                                T *__restrict__ inv) {
-    __shared__ T L[PNB][PNB];
-    const int j = threadIdx.x;
+__global__ __launch_bounds__(256) void k_tri_inv(const T *__restrict__ A, long lda, int j0, int jb,
+                                                 T *__restrict__ inv) {
+    using Acc = typename Mfma<T>::acc;
+    constexpr int H = PNB / 2;
+    __shared__ T L[PNB][PNB + 1];  // L, overwritten by L^-1 block by block
+    __shared__ T Tm[H][H + 1];     // L21 X11
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
     {
-        T v[PNB];
+        constexpr int PER = PNB * PNB / 256;
+        T v[PER];
 #pragma unroll
-        for (int r = 0; r < PNB; ++r) v[r] = (r < jb && j < jb) ? A[(long)(j0 + r) * lda + j0 + j] : T(0);
+        for (int q = 0; q < PER; ++q) {
+            const int e = q * 256 + tid, r = e >> 6, c = e & 63;
+            v[q] = (r < jb && c < jb) ? A[(long)(j0 + r) * lda + j0 + c] : T(0);
+        }
 #pragma unroll
-        for (int r = 0; r < PNB; ++r) L[r][j] = v[r];
+        for (int q = 0; q < PER; ++q) {
+            const int e = q * 256 + tid, r = e >> 6, c = e & 63;
+            L[r][c] = v[q];
+        }
     }
     __syncthreads();
-    T x[PNB];
+    if (wave < 2 && lane < H) {
+        // X11 (wave 0) or X22 (wave 1): every read of the block precedes the
+        // wave's writes to it, and the two blocks do not overlap
+        const int o = wave * H, j = lane;
+        T x[H];
 #pragma unroll
-    for (int i = 0; i < PNB; ++i) {
-        T s = (i == j) ? T(1) : T(0);
+        for (int i = 0; i < H; ++i) {
+            T s = (i == j) ? T(1) : T(0);
 #pragma unroll
-        for (int q = 0; q < i; ++q) s -= L[i][q] * x[q];
-        x[i] = s;
+            for (int q = 0; q < i; ++q) s -= L[o + i][o + q] * x[q];
+            x[i] = s;
+        }
+#pragma unroll
+        for (int i = 0; i < H; ++i) L[o + i][o + j] = x[i];  // zero above the diagonal
     }
+    __syncthreads();
+    const int tr = (wave >> 1) * 16, tc = (wave & 1) * 16;  // this wave's 16 x 16 tile
+    {   // Tm = L21 X11
+        Acc acc = Acc{0, 0, 0, 0};
 #pragma unroll
-    for (int i = 0; i < PNB; ++i) inv[i * PNB + j] = (i < jb && j < jb) ? x[i] : T(0);
+        for (int s = 0; s < H / 4; ++s)
+            acc = Mfma<T>::op(L[H + tr + lr][4 * s + lk], L[4 * s + lk][tc + lr], acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Tm[tr + Mfma<T>::row(lk, r)][tc + lr] = acc[r];
+    }
+    __syncthreads();
+    {   // X21 = -X22 Tm, over L21 (no longer read)
+        Acc acc = Acc{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < H / 4; ++s)
+            acc = Mfma<T>::op(L[H + tr + lr][H + 4 * s + lk], Tm[4 * s + lk][tc + lr], acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) L[H + tr + Mfma<T>::row(lk, r)][tc + lr] = -acc[r];
+    }
+    __syncthreads();
+    {
+        constexpr int PER = PNB * PNB / 256;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = q * 256 + tid, i = e >> 6, c = e & 63;
+            inv[e] = (i < jb && c < jb && c <= i) ? L[i][c] : T(0);
+        }
+    }
 }
 
 // B := inv * B for one 64-column slab per workgroup, in place: the slab is
@@ -726,19 +775,17 @@ static int panel(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, 
 
 template <class T>
 static int laswp(hipStream_t s, T *a, size_t lda, const int64_t *ipiv, size_t k0, size_t cnt,
-                 size_t c0, size_t c1, size_t c2, size_t c3, const LuWork &lw) {
+                 size_t c0, size_t c1, size_t c2, size_t c3) {
     const size_t ncols = (c1 - c0) + (c3 - c2);
     if (ncols == 0 || cnt == 0) return RLG_OK;
-    // algorithmic bytes: at most 2 cnt rows change, each read and written
-    RLG_PROF_W("laswp", s, 2.0 * 2.0 * (double)cnt * (double)ncols * sizeof(T));
     if (cnt > (size_t)ONB) {
         set_error("row swap block of %zu rows exceeds %d", cnt, ONB);
         return RLG_E_INVALID_ARG;
     }
-    hipLaunchKernelGGL(k_swap_map, dim3(1), dim3(ONB), 0, s, ipiv, (int)k0, (int)cnt, lw.map);
-    RLG_LAUNCHED();
-    hipLaunchKernelGGL(k_laswp<T>, dim3(ceil_div(ncols, LCOLS)), dim3(256), 0, s, a, (long)lda,
-                       lw.map, (int)c0, (int)c1, (int)c2, (int)c3);
+    // algorithmic bytes: at most 2 cnt rows change, each read and written
+    RLG_PROF_W("laswp", s, 2.0 * 2.0 * (double)cnt * (double)ncols * sizeof(T));
+    hipLaunchKernelGGL(k_laswp<T>, dim3(ceil_div(ncols, LCOLS)), dim3(ONB), 0, s, a, (long)lda, ipiv,
+                       (int)k0, (int)cnt, (int)c0, (int)c1, (int)c2, (int)c3);
     RLG_LAUNCHED();
     return RLG_OK;
 }
@@ -755,7 +802,7 @@ static int tri_inv(hipStream_t s, const T *a, size_t lda, size_t j0, size_t jb, 
     if (jb <= 1) return RLG_OK;
     // algorithmic bytes: the unit-lower block read, the 64 x 64 inverse written
     RLG_PROF_W("tri_inv", s, ((double)jb * jb / 2 + (double)PNB * PNB) * sizeof(T));
-    hipLaunchKernelGGL(k_tri_inv<T>, dim3(1), dim3(PNB), 0, s, a, (long)lda, (int)j0, (int)jb,
+    hipLaunchKernelGGL(k_tri_inv<T>, dim3(1), dim3(256), 0, s, a, (long)lda, (int)j0, (int)jb,
                        linv_of<T>(lw, j0));
     RLG_LAUNCHED();
     return RLG_OK;
@@ -815,7 +862,7 @@ static int outer_panel(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv,
         // L^-1 of the panel's diagonal block, for its two block-row solves
         if (j0 + jb < n) RLG_TRY(tri_inv<T>(s, a, lda, j0, jb, lw));
         // inner swaps on the other columns of the outer block
-        RLG_TRY(laswp<T>(s, a, lda, ipiv, j0, jb, k0, j0, j0 + jb, k0 + kb, lw));
+        RLG_TRY(laswp<T>(s, a, lda, ipiv, j0, jb, k0, j0, j0 + jb, k0 + kb));
         if (j0 + jb < k0 + kb) {
             RLG_TRY(trsm_llu<T>(s, a, lda, j0, jb, j0 + jb, k0 + kb, lw));
             RLG_TRY(update_native<T>(s, n - j0 - jb, k0 + kb - j0 - jb, jb, a + (j0 + jb) * lda + j0,
@@ -862,7 +909,7 @@ static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ip
             ev.big_pending = false;
         }
         // outer swaps on everything left and right of the outer block
-        RLG_TRY(laswp<T>(s, a, lda, ipiv, k0, kb, 0, k0, c0, n, lw));
+        RLG_TRY(laswp<T>(s, a, lda, ipiv, k0, kb, 0, k0, c0, n));
         if (c0 >= n) break;
         for (size_t j0 = k0; j0 < c0; j0 += PNB) {
             const size_t jb = c0 - j0 < (size_t)PNB ? c0 - j0 : PNB;

@@ -59,30 +59,34 @@ template <int MODE> __device__ __forceinline__ constexpr bool forward() {
 }
 template <int MODE> __device__ __forceinline__ constexpr bool stored_upper() { return MODE == UPPER; }
 
-// Stage one 64 x 64 tile of M (rows rb.., cols cb.., rs x cs valid) into
-// LDS; `diag` keeps only the stored triangle and zeroes the rest, so entries
-// of the other triangle are never read as numbers.
-// All 16 loads of a thread are issued before the first LDS store, so the
-// tile costs one memory latency rather than sixteen.
+// One 64 x 64 tile of M (rows rb.., cols cb.., rs x cs valid), 16 values per
+// thread: tile_load issues the loads, tile_store puts them in LDS.  `diag`
+// keeps only the stored triangle and zeroes the rest, so entries of the
+// other triangle are never read as numbers.
+constexpr int TPER = SB * SB / 256;
 template <class T, int MODE>
-__device__ __forceinline__ void stage_tile(T (*tl)[SB + 1], const T *__restrict__ M, long lda,
-                                           int rb, int cb, int rs, int cs, bool diag) {
-    constexpr int PER = SB * SB / 256;
-    T v[PER];
+__device__ __forceinline__ void tile_load(T (&v)[TPER], const T *__restrict__ M, long lda, int rb,
+                                          int cb, int rs, int cs, bool diag) {
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
+    for (int q = 0; q < TPER; ++q) {
         const int e = q * 256 + threadIdx.x;
         const int i = e >> 6, j = e & 63;
         bool ok = i < rs && j < cs;
         if (diag) ok = ok && (stored_upper<MODE>() ? j >= i : j <= i);
         v[q] = ok ? M[(long)(rb + i) * lda + cb + j] : T(0);
     }
+}
+template <class T>
+__device__ __forceinline__ void tile_store(T (*tl)[SB + 1], const T (&v)[TPER]) {
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
+    for (int q = 0; q < TPER; ++q) {
         const int e = q * 256 + threadIdx.x;
         tl[e >> 6][e & 63] = v[q];
     }
 }
+
+// Right-hand sides whose update-phase rows of W are loaded up front.
+constexpr int RPRE = 2;
 
 template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long lda, int n, int q0,
@@ -97,62 +101,115 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
     const int bs0 = min(SB, S), bs1 = S - bs0;  // rows of sub-blocks 0 and 1
     constexpr bool fwd = forward<MODE>();
 
-    stage_tile<T, MODE>(tl[0], M, lda, s0, s0, bs0, bs0, true);
-    if (nq > 1) {
-        stage_tile<T, MODE>(tl[1], M, lda, s0 + SB, s0 + SB, bs1, bs1, true);
-        if (stored_upper<MODE>())
-            stage_tile<T, MODE>(tl[2], M, lda, s0, s0 + SB, bs0, bs1, false);
-        else
-            stage_tile<T, MODE>(tl[2], M, lda, s0 + SB, s0, bs1, bs0, false);
-    }
-    __syncthreads();
-
-    // ---- 1. the super-block, redundantly in every workgroup
-    for (int r = wave; r < nrhs; r += 4) {
-#pragma clang fp contract(off)
-        T xq0 = T(0), xq1 = T(0);
-        for (int qi = 0; qi < nq; ++qi) {
-            const int q = fwd ? qi : nq - 1 - qi;
-            const int bs = q == 0 ? bs0 : bs1;
-            T x = lane < bs ? W[(long)r * ldw + s0 + q * SB + lane] : T(0);
-            // The loops below run all 64 steps, unrolled, so the LDS reads of
-            // the tile are issued ahead of the serial chain.  Steps beyond the
-            // block size change nothing: the staged tiles are zero outside the
-            // block and x is zero on lanes beyond it, so they subtract exact
-            // zeros.
-            if (qi == 1) {  // contribution of the sub-block solved first
-                const T xp = fwd ? xq0 : xq1;
+    // ---- every read that does not depend on the super-block's solution is
+    // issued first, so a launch waits for memory once, not once per phase:
+    // the update slice and, for the first right-hand sides, its rows of W;
+    // the three tiles; this wave's first right-hand side on the super-block.
+    constexpr int NIT = RPW / 16, NC = TSB * SB / 16;  // row forms
+    constexpr int CW = TSB * SB / 4;                   // transposed form
+    const int ls = lane & 15, grp = lane >> 4;
+    const int rbase = u0 + (int)blockIdx.x * RPW + wave * (RPW / 4) + grp;
+    const int jt = u0 + (int)blockIdx.x * CPW + lane;
+    T mv[MODE == LOWER_T ? CW : NIT * NC];
+    T wpre[RPRE][MODE == LOWER_T ? 1 : NIT];
+    if constexpr (MODE == LOWER_T) {
+        const bool act = jt < u1;
 #pragma unroll
-                for (int c = 0; c < SB; ++c) {
-                    const T xc = lane_val(xp, c);
-                    const T m = MODE == LOWER_T ? tl[2][c][lane] : tl[2][lane][c];
-                    x = x - m * xc;
-                }
-            }
-            const T(*d)[SB + 1] = tl[q];
-            const T dl = lane < bs ? d[lane][lane] : T(1);
-            const T rd = T(1) / dl;  // used by the FP32 form only
-            if (fwd) {
-#pragma unroll
-                for (int c = 0; c < SB; ++c) {
-                    if (MODE == LOWER && lane == c) x = diag_div(x, dl, rd);
-                    const T xc = lane_val(x, c);
-                    if (lane > c) x = x - d[lane][c] * xc;
-                }
-            } else {
-#pragma unroll
-                for (int cc = 0; cc < SB; ++cc) {
-                    const int c = SB - 1 - cc;
-                    if (lane == c) x = diag_div(x, dl, rd);
-                    const T xc = lane_val(x, c);
-                    if (lane < c) x = x - (MODE == UPPER ? d[lane][c] : d[c][lane]) * xc;
-                }
-            }
-            if (lane >= bs) x = T(0);
-            if (q == 0) xq0 = x; else xq1 = x;
+        for (int q = 0; q < CW; ++q) {
+            const int c = wave * CW + q;
+            mv[q] = (act && c < S) ? M[(long)(s0 + c) * lda + jt] : T(0);
         }
-        xs[r][lane] = xq0;
-        xs[r][SB + lane] = xq1;
+#pragma unroll
+        for (int r = 0; r < RPRE; ++r)
+            wpre[r][0] = (wave == 0 && act && r < nrhs) ? W[(long)r * ldw + jt] : T(0);
+    } else {
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int j = rbase + it * 4;
+            const T *mrow = M + (long)j * lda + s0;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                const int c = ls + 16 * k;
+                mv[it * NC + k] = (j < u1 && c < S) ? mrow[c] : T(0);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPRE; ++r)
+#pragma unroll
+            for (int it = 0; it < NIT; ++it) {
+                const int j = rbase + it * 4;
+                wpre[r][it] = (ls == 0 && j < u1 && r < nrhs) ? W[(long)r * ldw + j] : T(0);
+            }
+    }
+    {
+        T v0[TPER], v1[TPER], v2[TPER];
+        tile_load<T, MODE>(v0, M, lda, s0, s0, bs0, bs0, true);
+        if (nq > 1) {
+            tile_load<T, MODE>(v1, M, lda, s0 + SB, s0 + SB, bs1, bs1, true);
+            if (stored_upper<MODE>())
+                tile_load<T, MODE>(v2, M, lda, s0, s0 + SB, bs0, bs1, false);
+            else
+                tile_load<T, MODE>(v2, M, lda, s0 + SB, s0, bs1, bs0, false);
+        }
+        // this wave's first right-hand side on the two sub-blocks
+        const bool r0ok = wave < nrhs;
+        const T xw0 = (r0ok && lane < bs0) ? W[(long)wave * ldw + s0 + lane] : T(0);
+        const T xw1 = (r0ok && nq > 1 && lane < bs1) ? W[(long)wave * ldw + s0 + SB + lane] : T(0);
+        tile_store(tl[0], v0);
+        if (nq > 1) {
+            tile_store(tl[1], v1);
+            tile_store(tl[2], v2);
+        }
+        __syncthreads();
+
+        // ---- 1. the super-block, redundantly in every workgroup
+        for (int r = wave; r < nrhs; r += 4) {
+#pragma clang fp contract(off)
+            T xq0 = T(0), xq1 = T(0);
+            for (int qi = 0; qi < nq; ++qi) {
+                const int q = fwd ? qi : nq - 1 - qi;
+                const int bs = q == 0 ? bs0 : bs1;
+                T x = r == wave ? (q == 0 ? xw0 : xw1)
+                                : (lane < bs ? W[(long)r * ldw + s0 + q * SB + lane] : T(0));
+                // The loops below run all 64 steps, unrolled, so the LDS reads of
+                // the tile are issued ahead of the serial chain.  Steps beyond the
+                // block size change nothing: the staged tiles are zero outside the
+                // block and x is zero on lanes beyond it, so they subtract exact
+                // zeros.
+                if (qi == 1) {  // contribution of the sub-block solved first
+                    const T xp = fwd ? xq0 : xq1;
+#pragma unroll
+                    for (int c = 0; c < SB; ++c) {
+                        const T xc = lane_val(xp, c);
+                        const T m = MODE == LOWER_T ? tl[2][c][lane] : tl[2][lane][c];
+                        x = x - m * xc;
+                    }
+                }
+                const T(*d)[SB + 1] = tl[q];
+                const T dl = lane < bs ? d[lane][lane] : T(1);
+                const T rd = T(1) / dl;  // used by the FP32 form only
+                if (fwd) {
+#pragma unroll
+                    for (int c = 0; c < SB; ++c) {
+                        if (MODE == LOWER && lane == c) x = diag_div(x, dl, rd);
+                        const T xc = lane_val(x, c);
+                        if (lane > c) x = x - d[lane][c] * xc;
+                    }
+                } else {
+#pragma unroll
+                    for (int cc = 0; cc < SB; ++cc) {
+                        const int c = SB - 1 - cc;
+                        if (lane == c) x = diag_div(x, dl, rd);
+                        const T xc = lane_val(x, c);
+                        if (lane < c) x = x - (MODE == UPPER ? d[lane][c] : d[c][lane]) * xc;
+                    }
+                }
+                if (lane >= bs) x = T(0);
+                if (q == 0) xq0 = x; else xq1 = x;
+            }
+            xs[r][lane] = xq0;
+            xs[r][SB + lane] = xq1;
+        }
     }
     __syncthreads();
     if (blockIdx.x == 0)
@@ -162,20 +219,10 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
         }
 
     // ---- 2. remove the solved rows from this workgroup's share of [u0, u1)
-    // Every thread issues all of its matrix loads before the first use, so a
-    // workgroup pays one memory latency per phase, not one per row.
-    if (MODE == LOWER_T) {
+    if constexpr (MODE == LOWER_T) {
         // W[r][j] -= sum_c L[s0 + c][j] * x[c]: a workgroup owns 64 entries j
         // (one per lane: coalesced rows of L), wave w sums c in [32w, 32w + 32)
-        constexpr int CW = TSB * SB / 4;
-        const int j = u0 + (int)blockIdx.x * CPW + lane;
-        const bool act = j < u1;
-        T mv[CW];
-#pragma unroll
-        for (int q = 0; q < CW; ++q) {
-            const int c = wave * CW + q;
-            mv[q] = (act && c < S) ? M[(long)(s0 + c) * lda + j] : T(0);
-        }
+        const bool act = jt < u1;
         // partial sums meet in LDS (the staged tiles are no longer read)
         T(*red)[TMAXR][SB] = reinterpret_cast<T(*)[TMAXR][SB]>(&tl[0][0][0]);
         for (int r = 0; r < nrhs; ++r) {
@@ -188,33 +235,20 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
         if (wave == 0 && act) {
             for (int r = 0; r < nrhs; ++r) {
                 const T a = (red[0][r][lane] + red[1][r][lane]) + (red[2][r][lane] + red[3][r][lane]);
-                W[(long)r * ldw + j] -= a;
+                const T w0 = r < RPRE ? wpre[r < RPRE ? r : 0][0] : W[(long)r * ldw + jt];
+                W[(long)r * ldw + jt] = w0 - a;
             }
         }
     } else {
         // W[r][j] -= sum_c M[j][s0 + c] * x[c]; 16 lanes per row, 4 rows per
-        // wave at a time, lane ls reads columns ls, ls + 16, ... (coalesced)
-        constexpr int NIT = RPW / 16, NC = TSB * SB / 16;
-        const int ls = lane & 15, grp = lane >> 4;
-        const int rbase = u0 + (int)blockIdx.x * RPW + wave * (RPW / 4) + grp;
-        T mv[NIT][NC];
-#pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int j = rbase + it * 4;
-            const T *mrow = M + (long)j * lda + s0;
-#pragma unroll
-            for (int k = 0; k < NC; ++k) {
-                const int c = ls + 16 * k;
-                mv[it][k] = (j < u1 && c < S) ? mrow[c] : T(0);
-            }
-        }
+        // wave at a time, lane ls holds columns ls, ls + 16, ... (coalesced)
         for (int r = 0; r < nrhs; ++r) {  // wave-uniform
             T v[NIT];
 #pragma unroll
             for (int it = 0; it < NIT; ++it) {
                 T a = T(0);
 #pragma unroll
-                for (int k = 0; k < NC; ++k) a += mv[it][k] * xs[r][ls + 16 * k];
+                for (int k = 0; k < NC; ++k) a += mv[it * NC + k] * xs[r][ls + 16 * k];
                 a += __shfl_xor(a, 8, 64);
                 a += __shfl_xor(a, 4, 64);
                 a += __shfl_xor(a, 2, 64);
@@ -226,7 +260,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
 #pragma unroll
                 for (int it = 0; it < NIT; ++it) {
                     const int j = rbase + it * 4;
-                    w[it] = j < u1 ? W[(long)r * ldw + j] : T(0);
+                    w[it] = r < RPRE ? wpre[r < RPRE ? r : 0][it] : (j < u1 ? W[(long)r * ldw + j] : T(0));
                 }
 #pragma unroll
                 for (int it = 0; it < NIT; ++it) {
