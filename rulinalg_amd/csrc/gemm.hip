@@ -31,13 +31,16 @@ constexpr int GBM = 128, GBN = 128, GBK = 16, GTHREADS = 256;
 // dispatched round-robin over the 8 XCDs; regrouping gives each XCD a
 // contiguous range of tiles, and within a range tiles go down columns of
 // GROUP_M tile-rows, so that an XCD's L2 sees reuse of both A and B panels.
+// The lower-triangle form does not regroup: its tiles above the diagonal
+// return at once, so contiguous ranges would hand the XCDs unequal work.
+template <bool LOWER>
 __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, int &tn) {
     const int total = tiles_m * tiles_n;
     int id = blockIdx.x;
     // the first 8 * floor(total / 8) ids are regrouped; the few left over
     // keep their own tile
     const int full = total & ~7;
-    if (total >= 64 && id < full) {
+    if (!LOWER && total >= 64 && id < full) {
         const int per_xcd = full / 8;
         id = (id % 8) * per_xcd + id / 8;
     }
@@ -69,7 +72,7 @@ __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
     __shared__ T Bs[2][BK][GBN + (TRANSB ? 1 : 0)];
 
     int tm, tn;
-    tile_coords(tiles_m, tiles_n, tm, tn);
+    tile_coords<LOWER>(tiles_m, tiles_n, tm, tn);
     if (LOWER && tn > tm) return;  // strictly above the diagonal
     const int m0 = tm * GBM, n0 = tn * GBN;
     const int tid = threadIdx.x;
@@ -228,7 +231,7 @@ __global__ __launch_bounds__(GTHREADS) void k_gemm_lowp(
     __shared__ __attribute__((aligned(16))) H Bs[2][GBN][LBK + LPAD];
 
     int tm, tn;
-    tile_coords(tiles_m, tiles_n, tm, tn);
+    tile_coords<LOWER>(tiles_m, tiles_n, tm, tn);
     if (LOWER && tn > tm) return;
     const int m0 = tm * GBM, n0 = tn * GBN;
     const int tid = threadIdx.x;
@@ -367,7 +370,7 @@ __global__ __launch_bounds__(GTHREADS) void k_gemm_nt_lowp(
     __shared__ __attribute__((aligned(16))) H Bs[2][GBN][LBK + LPAD];
 
     int tm, tn;
-    tile_coords(tiles_m, tiles_n, tm, tn);
+    tile_coords<LOWER>(tiles_m, tiles_n, tm, tn);
     if (LOWER && tn > tm) return;
     const int m0 = tm * GBM, n0 = tn * GBN;
     const int tid = threadIdx.x;
