@@ -270,13 +270,6 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
     __shared__ T red_v[NW];
     __shared__ int red_i[NW];
     __shared__ int s_piv[PNB];
-    // the all-wave exchange read (phase D): each wave's best candidate, its
-    // row, row j, whether the wave's reads completed, wave 0's state
-    __shared__ T sp_row[NW][PNB];
-    __shared__ T sp_diag[PNB];
-    __shared__ T sp_v[NW];
-    __shared__ int sp_i[NW], sp_q[NW], sp_ok[NW];
-    __shared__ int s_live;
 
     const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
     const int r0 = g * PRT;
@@ -362,7 +355,6 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
                     xput_val(mine + 1, tag, bv);
                 }
             }
-            if (lane == 0) s_live = live;
         }
 
         // C. pivot j - 1 on the columns right of j
@@ -375,94 +367,11 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
         }
         __syncthreads();
 
-        // D. the winner, the swap and the pivot row.  With at most NW * SPW
-        // workgroups every wave takes part in the read: wave w polls the
-        // headers and the whole rows of its share of the slots (wave 0 also
-        // row j's slot) in one round trip, so the winning row is in hand once
-        // the winner is known instead of one more round trip after that.
-        constexpr int SPW = 16;
-        const bool spec = G > 1 && G <= NW * SPW && s_live;
-        if (spec) {
-            const unsigned tag = (unsigned)(k + j + 1);
-            const xword *base = xch + (size_t)(j & 1) * XPAR * XS;
-            const int spw = (G + NW - 1) / NW, qa = wave * spw;
-            const int qn = max(0, min(G, qa + spw) - qa);  // this wave's slots
-            T rv[SPW];
-#pragma unroll
-            for (int si = 0; si < SPW; ++si) rv[si] = T(0);
-            T dv = T(0), hv = T(0);
-            int hi = INT_MAX;
-            bool lv = true;
-            for (unsigned spins = 0;;) {
-                bool ok = true;
-#pragma unroll
-                for (int si = 0; si < SPW; ++si) {
-                    if (si < qn && lane < w) {
-                        bool o;
-                        rv[si] = xget_val<T>(base + (size_t)(qa + si) * XS + XHDR + lane * VW, tag, o);
-                        ok = ok && o;
-                    }
-                }
-                if (lane < qn) {
-                    const xword *h = base + (size_t)(qa + lane) * XS;
-                    const xword hr = xget(h);
-                    bool o;
-                    hv = xget_val<T>(h + 1, tag, o);
-                    hi = (int)(unsigned)hr;
-                    ok = ok && o && xtag(hr) == tag;
-                }
-                if (wave == 0 && lane < w) {
-                    bool o;
-                    dv = xget_val<T>(base + (size_t)GMAX * XS + XHDR + lane * VW, tag, o);
-                    ok = ok && o;
-                }
-                if (__all(ok)) break;
-                if (!xwait(spins, timeout)) { lv = false; break; }
-            }
-            if (lane >= qn) hi = INT_MAX;
-            const int wl = wave_first_max_lane(hv, hi);  // = the slot's index in the wave's share
-            T rowv = rv[0];
-#pragma unroll
-            for (int si = 1; si < SPW; ++si) rowv = si == wl ? rv[si] : rowv;
-            const T wv = wl >= 0 ? lane_get(hv, wl) : T(0);
-            const int wi = wl >= 0 ? lane_get(hi, wl) : INT_MAX;
-            sp_row[wave][lane] = rowv;
-            if (wave == 0) sp_diag[lane] = dv;
-            if (lane == 0) {
-                sp_v[wave] = wv;
-                sp_i[wave] = wi;
-                sp_q[wave] = qa + wl;
-                sp_ok[wave] = lv;
-            }
-            __syncthreads();
-        }
+        // D. the winner, the swap and the pivot row
         if (wave == 0) {
             int p = bi;
             T pabs = bv;
-            if (spec) {
-                T v = T(0);
-                int i = INT_MAX, bw = 0;
-                bool all_ok = true;
-#pragma unroll
-                for (int q = 0; q < NW; ++q) {
-                    const bool take = better(sp_v[q], sp_i[q], v, i);
-                    v = take ? sp_v[q] : v;
-                    i = take ? sp_i[q] : i;
-                    bw = take ? q : bw;
-                    all_ok = all_ok && sp_ok[q];
-                }
-                if (!all_ok || i == INT_MAX) {
-                    if (all_ok && lane == 0)  // no candidate anywhere: cannot happen; report it
-                        __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    live = false;
-                } else {
-                    p = i;
-                    pabs = v;
-                    pv = sp_row[bw][lane];
-                    if (p != j && p >= r0 && p < r0 + rows && !own_j) oj = sp_diag[lane];
-                }
-                if (!live) { p = bi; pabs = bv; }
-            } else if (G > 1 && live) {
+            if (G > 1 && live) {
                 const unsigned tag = (unsigned)(k + j + 1);
                 const xword *base = xch + (size_t)(j & 1) * XPAR * XS;
                 const xword *diag = base + (size_t)GMAX * XS;

@@ -25,6 +25,7 @@
 // (src/matrix/mod.rs:318-398, lu.rs:624-642) form row dot products instead;
 // the results agree to rounding.
 #include "kernels.hpp"
+#include "wave.hpp"
 
 namespace rlg {
 
@@ -35,18 +36,6 @@ constexpr int RPW = 64;     // rows updated per workgroup (row-access forms)
 constexpr int CPW = 64;     // entries updated per workgroup (transposed form)
 
 enum TriMode { LOWER_UNIT = 0, LOWER = 1, UPPER = 2, LOWER_T = 3 };
-
-// x of lane `src` to every lane; `src` is wave-uniform, so this is a
-// v_readlane (scalar result, a few cycles) rather than an LDS permute.
-__device__ __forceinline__ float lane_val(float x, int src) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
-}
-__device__ __forceinline__ double lane_val(double x, int src) {
-    const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), src);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
 
 // x / d on the solve's serial chain: FP64 divides (the reference divides);
 // FP32 (the refinement's correction solves) multiplies by a reciprocal that
@@ -180,7 +169,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
                     const T xp = fwd ? xq0 : xq1;
 #pragma unroll
                     for (int c = 0; c < SB; ++c) {
-                        const T xc = lane_val(xp, c);
+                        const T xc = lane_get(xp, c);
                         const T m = MODE == LOWER_T ? tl[2][c][lane] : tl[2][lane][c];
                         x = x - m * xc;
                     }
@@ -192,7 +181,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
 #pragma unroll
                     for (int c = 0; c < SB; ++c) {
                         if (MODE == LOWER && lane == c) x = diag_div(x, dl, rd);
-                        const T xc = lane_val(x, c);
+                        const T xc = lane_get(x, c);
                         if (lane > c) x = x - d[lane][c] * xc;
                     }
                 } else {
@@ -200,7 +189,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
                     for (int cc = 0; cc < SB; ++cc) {
                         const int c = SB - 1 - cc;
                         if (lane == c) x = diag_div(x, dl, rd);
-                        const T xc = lane_val(x, c);
+                        const T xc = lane_get(x, c);
                         if (lane < c) x = x - (MODE == UPPER ? d[lane][c] : d[c][lane]) * xc;
                     }
                 }
@@ -249,11 +238,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
                 T a = T(0);
 #pragma unroll
                 for (int k = 0; k < NC; ++k) a += mv[it * NC + k] * xs[r][ls + 16 * k];
-                a += __shfl_xor(a, 8, 64);
-                a += __shfl_xor(a, 4, 64);
-                a += __shfl_xor(a, 2, 64);
-                a += __shfl_xor(a, 1, 64);
-                v[it] = a;
+                v[it] = row16_sum(a);  // the 16 lanes of a row are one DPP row
             }
             if (ls == 0) {
                 T w[NIT];

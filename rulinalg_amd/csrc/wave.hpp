@@ -60,6 +60,17 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long k)
     return mx(mx(lane_get(k, 0), lane_get(k, 16)), mx(lane_get(k, 32), lane_get(k, 48)));
 }
 
+// Sum over each row of 16 lanes, in every lane of the row: four rotations
+// within the row (row_ror 8, 4, 2, 1), each a DPP move.
+template <int N> constexpr int DPP_ROR = 0x120 + N;
+template <class T> __device__ __forceinline__ T row16_sum(T a) {
+    a += dpp<DPP_ROR<8>>(a);
+    a += dpp<DPP_ROR<4>>(a);
+    a += dpp<DPP_ROR<2>>(a);
+    a += dpp<DPP_ROR<1>>(a);
+    return a;
+}
+
 // Minimum of an int over the wave (uniform result), the same pattern.
 __device__ __forceinline__ int wave_min_i32(int k) {
     k = min(k, dpp<DPP_XOR1>(k));
