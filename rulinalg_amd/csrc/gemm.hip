@@ -59,7 +59,12 @@ __device__ __forceinline__ void tile_coords(int tiles_m, int tiles_n, int &tm, i
 template <class T> struct StageK { static constexpr int value = GBK; };
 template <> struct StageK<float> { static constexpr int value = 2 * GBK; };
 
-template <class T, bool TRANSB, bool LOWER>
+// VEC: operands 16-byte aligned with row strides, K (and N for the
+// untransposed B) multiples of 16 bytes' worth of elements: the tiles are
+// loaded with 16-byte loads (a quarter (FP32) or half (FP64) of the load
+// instructions), which measured 18 % faster on the LU's depth-256 FP32
+// update.  The untransposed B stage is padded by 16 bytes per row.
+template <class T, bool TRANSB, bool LOWER, bool VEC>
 __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
     int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
     const T *__restrict__ B, long ldb, T beta, T *__restrict__ C, long ldc,
@@ -67,9 +72,12 @@ __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
     using Acc = typename Mfma<T>::acc;
     constexpr int BK = StageK<T>::value;
     constexpr int PER = BK * GBM / GTHREADS;  // staged elements per thread per operand
-    static_assert(GBM == GBN && PER * GTHREADS == BK * GBM, "square tiles, whole stages");
+    constexpr int VL = 16 / sizeof(T);        // elements per 16-byte load
+    constexpr int VPER = PER / VL;            // 16-byte loads per thread per operand
+    static_assert(GBM == GBN && PER * GTHREADS == BK * GBM && VPER * VL == PER, "whole stages");
+    typedef T vT __attribute__((ext_vector_type(VL)));
     __shared__ T As[2][BK][GBM + 1];
-    __shared__ T Bs[2][BK][GBN + (TRANSB ? 1 : 0)];
+    __shared__ __attribute__((aligned(16))) T Bs[2][BK][GBN + (TRANSB ? 1 : VL)];
 
     int tm, tn;
     tile_coords<LOWER>(tiles_m, tiles_n, tm, tn);
@@ -86,36 +94,74 @@ __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
 
-    T ra[PER], rb[PER];
+    T ra[VEC ? 1 : PER], rb[VEC ? 1 : PER];
+    vT rav[VEC ? VPER : 1], rbv[VEC ? VPER : 1];
     auto load = [&](int k0) {
+        if constexpr (VEC) {
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int e = q * GTHREADS + tid;
-            {   // A tile, 128 x BK: e -> (i = e / BK, kk = e % BK)
-                const int i = e / BK, kk = e % BK;
-                const int gi = m0 + i, gk = k0 + kk;
-                ra[q] = (gi < M && gk < K) ? A[(long)gi * lda + gk] : T(0);
+            for (int q = 0; q < VPER; ++q) {
+                const int e = q * GTHREADS + tid;
+                {   // A tile, 128 x BK in BK / VL vectors per row
+                    const int i = e / (BK / VL), gk = k0 + (e % (BK / VL)) * VL, gi = m0 + i;
+                    rav[q] = (gi < M && gk < K) ? *reinterpret_cast<const vT *>(A + (long)gi * lda + gk)
+                                                : vT{};
+                }
+                if (!TRANSB) {  // B tile, BK x 128 in 128 / VL vectors per row
+                    const int gk = k0 + e / (GBN / VL), gj = n0 + (e % (GBN / VL)) * VL;
+                    rbv[q] = (gk < K && gj < N) ? *reinterpret_cast<const vT *>(B + (long)gk * ldb + gj)
+                                                : vT{};
+                } else {        // B^T given as N x K, vectors along k
+                    const int gj = n0 + e / (BK / VL), gk = k0 + (e % (BK / VL)) * VL;
+                    rbv[q] = (gj < N && gk < K) ? *reinterpret_cast<const vT *>(B + (long)gj * ldb + gk)
+                                                : vT{};
+                }
             }
-            if (!TRANSB) {  // B tile, BK x 128: e -> (kk = e / 128, j = e % 128)
-                const int kk = e / GBN, j = e % GBN;
-                const int gk = k0 + kk, gj = n0 + j;
-                rb[q] = (gk < K && gj < N) ? B[(long)gk * ldb + gj] : T(0);
-            } else {        // B^T given as N x K: e -> (j = e / BK, kk = e % BK)
-                const int j = e / BK, kk = e % BK;
-                const int gj = n0 + j, gk = k0 + kk;
-                rb[q] = (gj < N && gk < K) ? B[(long)gj * ldb + gk] : T(0);
+        } else {
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int e = q * GTHREADS + tid;
+                {   // A tile, 128 x BK: e -> (i = e / BK, kk = e % BK)
+                    const int i = e / BK, kk = e % BK;
+                    const int gi = m0 + i, gk = k0 + kk;
+                    ra[q] = (gi < M && gk < K) ? A[(long)gi * lda + gk] : T(0);
+                }
+                if (!TRANSB) {  // B tile, BK x 128: e -> (kk = e / 128, j = e % 128)
+                    const int kk = e / GBN, j = e % GBN;
+                    const int gk = k0 + kk, gj = n0 + j;
+                    rb[q] = (gk < K && gj < N) ? B[(long)gk * ldb + gj] : T(0);
+                } else {        // B^T given as N x K: e -> (j = e / BK, kk = e % BK)
+                    const int j = e / BK, kk = e % BK;
+                    const int gj = n0 + j, gk = k0 + kk;
+                    rb[q] = (gj < N && gk < K) ? B[(long)gj * ldb + gk] : T(0);
+                }
             }
         }
     };
     auto store = [&](int buf) {
+        if constexpr (VEC) {
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int e = q * GTHREADS + tid;
-            As[buf][e % BK][e / BK] = ra[q];
-            if (!TRANSB)
-                Bs[buf][e / GBN][e % GBN] = rb[q];
-            else
-                Bs[buf][e % BK][e / BK] = rb[q];
+            for (int q = 0; q < VPER; ++q) {
+                const int e = q * GTHREADS + tid;
+                const int i = e / (BK / VL), kb = (e % (BK / VL)) * VL;
+#pragma unroll
+                for (int v = 0; v < VL; ++v) As[buf][kb + v][i] = rav[q][v];
+                if (!TRANSB) {
+                    *reinterpret_cast<vT *>(&Bs[buf][e / (GBN / VL)][(e % (GBN / VL)) * VL]) = rbv[q];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < VL; ++v) Bs[buf][kb + v][i] = rbv[q][v];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int e = q * GTHREADS + tid;
+                As[buf][e % BK][e / BK] = ra[q];
+                if (!TRANSB)
+                    Bs[buf][e / GBN][e % GBN] = rb[q];
+                else
+                    Bs[buf][e % BK][e / BK] = rb[q];
+            }
         }
     };
 
@@ -498,14 +544,22 @@ static int gemm_impl(hipStream_t s, const char *cls, size_t m, size_t n, size_t 
     const double flops = (lower_only ? 1.0 : 2.0) * (double)m * (double)n * (double)k;
     static const int cls_id = prof_class(cls);
     ProfScope prof_scope(cls_id, s, flops);
-#define RLG_GEMM_LAUNCH(TB, LO)                                                              \
-    hipLaunchKernelGGL((k_gemm<T, TB, LO>), grid, block, 0, s, (int)m, (int)n, (int)k, alpha, \
+    // 16-byte loads when every vector the kernel reads is whole and aligned
+    constexpr size_t VL = 16 / sizeof(T);
+    auto al16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    const bool vec = k % VL == 0 && lda % VL == 0 && ldb % VL == 0 && al16(a) && al16(b) &&
+                     (trans_b || n % VL == 0);
+#define RLG_GEMM_LAUNCH(TB, LO, V)                                                               \
+    hipLaunchKernelGGL((k_gemm<T, TB, LO, V>), grid, block, 0, s, (int)m, (int)n, (int)k, alpha, \
                        a, (long)lda, b, (long)ldb, beta, c, (long)ldc, tiles_m, tiles_n)
+#define RLG_GEMM_LAUNCH_V(TB, LO) \
+    do { if (vec) RLG_GEMM_LAUNCH(TB, LO, true); else RLG_GEMM_LAUNCH(TB, LO, false); } while (0)
     if (trans_b) {
-        if (lower_only) RLG_GEMM_LAUNCH(true, true); else RLG_GEMM_LAUNCH(true, false);
+        if (lower_only) RLG_GEMM_LAUNCH_V(true, true); else RLG_GEMM_LAUNCH_V(true, false);
     } else {
-        if (lower_only) RLG_GEMM_LAUNCH(false, true); else RLG_GEMM_LAUNCH(false, false);
+        if (lower_only) RLG_GEMM_LAUNCH_V(false, true); else RLG_GEMM_LAUNCH_V(false, false);
     }
+#undef RLG_GEMM_LAUNCH_V
 #undef RLG_GEMM_LAUNCH
     RLG_LAUNCHED();
     return RLG_OK;

@@ -98,6 +98,20 @@ def test_gemm_f32_vs_oracle():
     assert np.linalg.norm(c - c_or) / (np.linalg.norm(a) * np.linalg.norm(b)) <= 1e-6
 
 
+@pytest.mark.parametrize("dt,m,k,n", [(np.float32, 131, 256, 260), (np.float32, 300, 36, 4),
+                                      (np.float64, 131, 258, 130), (np.float64, 257, 2, 514)])
+def test_gemm_vector_loads(dt, m, k, n):
+    """Shapes whose rows are whole 16-byte vectors (k and n multiples of 4 in
+    FP32, of 2 in FP64) take the kernel's 16-byte-load staging; m and n are
+    not multiples of the 128 x 128 tile."""
+    a = wl.uniform(m, k, 300 + m, dt)
+    b = wl.uniform(k, n, 400 + n, dt)
+    c = (rl.Matrix(a) * rl.Matrix(b)).to_numpy()
+    c_or = orc.matmul(a, b)
+    tol = 1e-6 if dt == np.float32 else 1e-13
+    assert np.linalg.norm(c - c_or) / (np.linalg.norm(a) * np.linalg.norm(b)) <= tol
+
+
 def test_gemm_dev_alpha_beta():
     lib = _lib.load()
     m, k, n = 150, 70, 90
