@@ -51,26 +51,81 @@ template <int MODE> __device__ __forceinline__ constexpr bool stored_upper() { r
 // One 64 x 64 tile of M (rows rb.., cols cb.., rs x cs valid), 16 values per
 // thread: tile_load issues the loads, tile_store puts them in LDS.  `diag`
 // keeps only the stored triangle and zeroes the rest, so entries of the
-// other triangle are never read as numbers.
+// other triangle are never read as numbers.  With `vec` (M and lda allow
+// 16-byte loads) a full tile is read with 16-byte loads, a quarter (FP32)
+// or half (FP64) of the load instructions: with the update slice also in
+// flight, scalar loads would exceed what a wave can have outstanding.
+// Staged rows are padded to 16-byte multiples (TP) so that a lane can read
+// its row back with 16-byte LDS reads.
 constexpr int TPER = SB * SB / 256;
+template <class T> constexpr int TVL = 16 / sizeof(T);  // elements per 16 bytes
+template <class T> constexpr int TP = TVL<T>;           // row padding
 template <class T, int MODE>
 __device__ __forceinline__ void tile_load(T (&v)[TPER], const T *__restrict__ M, long lda, int rb,
-                                          int cb, int rs, int cs, bool diag) {
+                                          int cb, int rs, int cs, bool diag, bool vec) {
+    constexpr int VL = TVL<T>;
+    typedef T vT __attribute__((ext_vector_type(VL)));
+    if (vec && rs == SB && cs == SB) {
 #pragma unroll
-    for (int q = 0; q < TPER; ++q) {
-        const int e = q * 256 + threadIdx.x;
-        const int i = e >> 6, j = e & 63;
-        bool ok = i < rs && j < cs;
-        if (diag) ok = ok && (stored_upper<MODE>() ? j >= i : j <= i);
-        v[q] = ok ? M[(long)(rb + i) * lda + cb + j] : T(0);
+        for (int u = 0; u < TPER / VL; ++u) {
+            const int e = u * 256 + threadIdx.x;  // vector index
+            const int i = e / (SB / VL), j0 = (e % (SB / VL)) * VL;
+            const vT x = *reinterpret_cast<const vT *>(M + (long)(rb + i) * lda + cb + j0);
+#pragma unroll
+            for (int t = 0; t < VL; ++t) {
+                const int j = j0 + t;
+                const bool ok = !diag || (stored_upper<MODE>() ? j >= i : j <= i);
+                v[u * VL + t] = ok ? x[t] : T(0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < TPER; ++q) {
+            const int e = q * 256 + threadIdx.x;
+            const int i = e >> 6, j = e & 63;
+            bool ok = i < rs && j < cs;
+            if (diag) ok = ok && (stored_upper<MODE>() ? j >= i : j <= i);
+            v[q] = ok ? M[(long)(rb + i) * lda + cb + j] : T(0);
+        }
     }
 }
 template <class T>
-__device__ __forceinline__ void tile_store(T (*tl)[SB + 1], const T (&v)[TPER]) {
+__device__ __forceinline__ void tile_store(T (*tl)[SB + TP<T>], const T (&v)[TPER], bool vec_full) {
+    constexpr int VL = TVL<T>;
+    if (vec_full) {
 #pragma unroll
-    for (int q = 0; q < TPER; ++q) {
-        const int e = q * 256 + threadIdx.x;
-        tl[e >> 6][e & 63] = v[q];
+        for (int u = 0; u < TPER / VL; ++u) {
+            const int e = u * 256 + threadIdx.x;
+            const int i = e / (SB / VL), j0 = (e % (SB / VL)) * VL;
+#pragma unroll
+            for (int t = 0; t < VL; ++t) tl[i][j0 + t] = v[u * VL + t];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < TPER; ++q) {
+            const int e = q * 256 + threadIdx.x;
+            tl[e >> 6][e & 63] = v[q];
+        }
+    }
+}
+
+// Row `lane` of a staged tile into registers (r[c] = t[lane][c]), or for the
+// transposed form its column `lane` (r[c] = t[c][lane]): the serial chain of
+// the substitution then reads registers, not LDS.
+template <class T, int MODE>
+__device__ __forceinline__ void tile_row(const T (*t)[SB + TP<T>], int lane, T (&r)[SB]) {
+    constexpr int VL = TVL<T>;
+    typedef T vT __attribute__((ext_vector_type(VL)));
+    if (MODE == LOWER_T) {
+#pragma unroll
+        for (int c = 0; c < SB; ++c) r[c] = t[c][lane];
+    } else {
+#pragma unroll
+        for (int c = 0; c < SB; c += VL) {
+            const vT x = *reinterpret_cast<const vT *>(&t[lane][c]);
+#pragma unroll
+            for (int q = 0; q < VL; ++q) r[c + q] = x[q];
+        }
     }
 }
 
@@ -80,10 +135,11 @@ constexpr int RPRE = 2;
 template <class T, int MODE>
 __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long lda, int n, int q0,
                                                    int nq, int nrhs, T *__restrict__ W, long ldw,
-                                                   T *__restrict__ X, long ldx, int u0, int u1) {
+                                                   T *__restrict__ X, long ldx, int u0, int u1,
+                                                   int vec) {
     // tiles: [0] diagonal of sub-block 0, [1] diagonal of sub-block 1,
     // [2] the off-diagonal tile inside the stored triangle
-    __shared__ T tl[3][SB][SB + 1];
+    __shared__ __attribute__((aligned(16))) T tl[3][SB][SB + TP<T>];
     __shared__ T xs[TMAXR][TSB * SB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int s0 = q0 * SB, s1 = min(n, (q0 + nq) * SB), S = s1 - s0;
@@ -132,22 +188,23 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
     }
     {
         T v0[TPER], v1[TPER], v2[TPER];
-        tile_load<T, MODE>(v0, M, lda, s0, s0, bs0, bs0, true);
+        const bool vf0 = vec && bs0 == SB, vf1 = vec && bs1 == SB;  // full tiles
+        tile_load<T, MODE>(v0, M, lda, s0, s0, bs0, bs0, true, vec);
         if (nq > 1) {
-            tile_load<T, MODE>(v1, M, lda, s0 + SB, s0 + SB, bs1, bs1, true);
+            tile_load<T, MODE>(v1, M, lda, s0 + SB, s0 + SB, bs1, bs1, true, vec);
             if (stored_upper<MODE>())
-                tile_load<T, MODE>(v2, M, lda, s0, s0 + SB, bs0, bs1, false);
+                tile_load<T, MODE>(v2, M, lda, s0, s0 + SB, bs0, bs1, false, vec);
             else
-                tile_load<T, MODE>(v2, M, lda, s0 + SB, s0, bs1, bs0, false);
+                tile_load<T, MODE>(v2, M, lda, s0 + SB, s0, bs1, bs0, false, vec);
         }
         // this wave's first right-hand side on the two sub-blocks
         const bool r0ok = wave < nrhs;
         const T xw0 = (r0ok && lane < bs0) ? W[(long)wave * ldw + s0 + lane] : T(0);
         const T xw1 = (r0ok && nq > 1 && lane < bs1) ? W[(long)wave * ldw + s0 + SB + lane] : T(0);
-        tile_store(tl[0], v0);
+        tile_store(tl[0], v0, vf0);
         if (nq > 1) {
-            tile_store(tl[1], v1);
-            tile_store(tl[2], v2);
+            tile_store(tl[1], v1, vf1);
+            tile_store(tl[2], v2, vf0 && vf1);
         }
         __syncthreads();
 
@@ -167,14 +224,17 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
                 // zeros.
                 if (qi == 1) {  // contribution of the sub-block solved first
                     const T xp = fwd ? xq0 : xq1;
+                    T m[SB];
+                    tile_row<T, MODE>(tl[2], lane, m);
 #pragma unroll
                     for (int c = 0; c < SB; ++c) {
                         const T xc = lane_get(xp, c);
-                        const T m = MODE == LOWER_T ? tl[2][c][lane] : tl[2][lane][c];
-                        x = x - m * xc;
+                        x = x - m[c] * xc;
                     }
                 }
-                const T(*d)[SB + 1] = tl[q];
+                const T(*d)[SB + TP<T>] = tl[q];
+                T dr[SB];  // this lane's row (column for the transposed form) of d
+                tile_row<T, MODE>(d, lane, dr);
                 const T dl = lane < bs ? d[lane][lane] : T(1);
                 const T rd = T(1) / dl;  // used by the FP32 form only
                 if (fwd) {
@@ -182,7 +242,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
                     for (int c = 0; c < SB; ++c) {
                         if (MODE == LOWER && lane == c) x = diag_div(x, dl, rd);
                         const T xc = lane_get(x, c);
-                        if (lane > c) x = x - d[lane][c] * xc;
+                        if (lane > c) x = x - dr[c] * xc;
                     }
                 } else {
 #pragma unroll
@@ -190,7 +250,7 @@ __global__ __launch_bounds__(256) void k_trsv_step(const T *__restrict__ M, long
                         const int c = SB - 1 - cc;
                         if (lane == c) x = diag_div(x, dl, rd);
                         const T xc = lane_get(x, c);
-                        if (lane < c) x = x - (MODE == UPPER ? d[lane][c] : d[c][lane]) * xc;
+                        if (lane < c) x = x - dr[c] * xc;
                     }
                 }
                 if (lane >= bs) x = T(0);
@@ -266,6 +326,8 @@ static int trsv_run(hipStream_t s, size_t n, const T *m, size_t lda, size_t nrhs
         return RLG_E_INVALID_ARG;
     }
     constexpr bool fwd = MODE == LOWER_UNIT || MODE == LOWER;
+    // 16-byte tile loads when rows of the triangle start 16-byte aligned
+    const bool vec = lda % TVL<T> == 0 && (reinterpret_cast<uintptr_t>(m) & 15) == 0;
     const size_t nb = (n + SB - 1) / SB;
     const size_t steps = (nb + TSB - 1) / TSB;
     for (size_t r0 = 0; r0 < nrhs; r0 += TMAXR) {
@@ -292,7 +354,7 @@ static int trsv_run(hipStream_t s, size_t n, const T *m, size_t lda, size_t nrhs
             RLG_PROF_W("trsv", s, bytes);
             hipLaunchKernelGGL((k_trsv_step<T, MODE>), dim3(grid), dim3(256), 0, s, m, (long)lda,
                                (int)n, (int)q0, (int)nq, (int)nr, w + r0 * ldw, (long)ldw,
-                               x + r0 * ldx, (long)ldx, (int)u0, (int)u1);
+                               x + r0 * ldx, (long)ldx, (int)u0, (int)u1, vec ? 1 : 0);
             RLG_LAUNCHED();
         }
     }
