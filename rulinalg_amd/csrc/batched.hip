@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "wave.hpp"
 
 namespace rlg {
 
@@ -69,9 +70,78 @@ struct Smem {
     int red_i[BTH / 64];
     int piv[32];
     double div[32];
+    int mdst[64], msrc[64];  // net row moves of a panel's swaps
+    int mcnt;
     int info;
     int item;
 };
+
+constexpr int BNW = BTH / 64;  // waves per workgroup
+constexpr int TSTRIP = 4;      // 16 x 16 tiles per wave step of the trailing updates
+
+// Copy the m x kb block at A[k.., k..] (ld n) into the LDS panel P (ld LD), or
+// back (TO_A); eight loads in flight per thread before their stores.  With
+// LOWER only the lower triangle of the top kb x kb block is moved.
+template <class T, int LD, bool TO_A, bool LOWER>
+__device__ __forceinline__ void panel_move(T *__restrict__ A, int n, int k, int m, int kb,
+                                           T *__restrict__ P) {
+    const int tot = m * kb, tid = threadIdx.x;
+    for (int e0 = tid; e0 < tot; e0 += 8 * BTH) {
+        T v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = e0 + q * BTH, r = e / kb, c = e - r * kb;
+            const bool ok = e < tot && (!LOWER || r >= kb || c <= r);
+            if (TO_A) v[q] = ok ? P[r * LD + c] : T(0);
+            else v[q] = ok ? A[(long)(k + r) * n + k + c] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = e0 + q * BTH, r = e / kb, c = e - r * kb;
+            const bool ok = e < tot && (!LOWER || r >= kb || c <= r);
+            if (ok) {
+                if (TO_A) A[(long)(k + r) * n + k + c] = v[q];
+                else P[r * LD + c] = v[q];
+            }
+        }
+    }
+}
+
+// Net effect of a panel's swaps (swap s exchanges local rows s and piv[s] >=
+// s, s < kb) as moves (dst, src): afterwards row dst holds what row src held
+// before.  Row t < kb is final after swap t and holds what row piv[t] held
+// just before it; a row x beyond the block is final after the last swap that
+// names it, and then holds what row s held just before swap s.  "What row x
+// held before swap t" is found by scanning the earlier swaps downwards.
+// Called by every thread; wave 0 (kb <= 32 lanes) fills sm.mdst / sm.msrc.
+__device__ __forceinline__ int bchase(const int *piv, int x, int t) {
+    for (int s = t - 1; s >= 0; --s)
+        if (piv[s] == x) x = s;
+    return x;
+}
+__device__ __forceinline__ void panel_moves(int kb, const int *piv, int *mdst, int *msrc, int *mcnt) {
+    const int t = threadIdx.x;
+    if (t < 64) {
+        int d_in = -1, s_in = -1, d_out = -1, s_out = -1;
+        if (t < kb) {
+            const int pt = piv[t];
+            const int si = bchase(piv, pt, t);
+            if (si != t) { d_in = t; s_in = si; }
+            if (pt >= kb) {
+                bool last = true;
+                for (int s = t + 1; s < kb; ++s) last = last && piv[s] != pt;
+                if (last) { d_out = pt; s_out = bchase(piv, t, t); }
+            }
+        }
+        const unsigned long long b_in = __ballot(d_in >= 0), b_out = __ballot(d_out >= 0);
+        const int n_in = __popcll(b_in);
+        const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b_in >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)b_in, 0));
+        const int below_o = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b_out >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)b_out, 0));
+        if (d_in >= 0) { mdst[below] = d_in; msrc[below] = s_in; }
+        if (d_out >= 0) { mdst[n_in + below_o] = d_out; msrc[n_in + below_o] = s_out; }
+        if (t == 0) *mcnt = n_in + __popcll(b_out);
+    }
+}
 
 template <class T>
 __device__ __forceinline__ bool bbetter(T v2, int i2, T v1, int i1) {
@@ -98,10 +168,7 @@ __device__ int lu_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
     __syncthreads();
     for (int k = 0; k < n; k += NB) {
         const int kb = min(NB, n - k), m = n - k;
-        for (int e = tid; e < m * kb; e += BTH) {
-            const int r = e / kb, c = e - r * kb;
-            P[r * LD + c] = A[(long)(k + r) * n + k + c];
-        }
+        panel_move<T, LD, false, false>(A, n, k, m, kb, P);
         __syncthreads();
         for (int j = 0; j < kb; ++j) {
             T bv = T(0);
@@ -146,33 +213,28 @@ __device__ int lu_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
             }
             __syncthreads();
         }
-        for (int e = tid; e < m * kb; e += BTH) {
-            const int r = e / kb, c = e - r * kb;
-            A[(long)(k + r) * n + k + c] = P[r * LD + c];
-        }
-        // row swaps on the columns outside the panel and on the RHS
+        panel_move<T, LD, true, false>(A, n, k, m, kb, P);
+        // row swaps on the columns outside the panel and on the RHS, as the
+        // net moves of the panel's swaps: every source read before any write
+        panel_moves(kb, sm.piv, sm.mdst, sm.msrc, &sm.mcnt);
+        __syncthreads();
         const int right = n - k - kb;
         const int ncols = k + right + nrhs;
-        for (int q = tid; q < ncols; q += BTH) {
-            for (int s = 0; s < kb; ++s) {
-                const int p = sm.piv[s];
-                if (p == s) continue;
-                T *x, *y;
-                if (q < k) {
-                    x = A + (long)(k + s) * n + q;
-                    y = A + (long)(k + p) * n + q;
-                } else if (q < k + right) {
-                    const int c = q - k + k + kb;
-                    x = A + (long)(k + s) * n + c;
-                    y = A + (long)(k + p) * n + c;
-                } else {
-                    const int r = q - k - right;
-                    x = B + (long)r * n + k + s;
-                    y = B + (long)r * n + k + p;
-                }
-                const T t = *x;
-                *x = *y;
-                *y = t;
+        const int mc = sm.mcnt;
+        if (mc > 0) {
+            for (int q = tid; q < ncols; q += BTH) {
+                T *base;
+                long rs;
+                if (q < k) { base = A + (long)k * n + q; rs = n; }
+                else if (q < k + right) { base = A + (long)k * n + kb + q; rs = n; }
+                else { base = B + (long)(q - k - right) * n + k; rs = 1; }
+                T v[2 * NB];
+#pragma unroll
+                for (int e = 0; e < 2 * NB; ++e)
+                    if (e < mc) v[e] = base[sm.msrc[e] * rs];
+#pragma unroll
+                for (int e = 0; e < 2 * NB; ++e)
+                    if (e < mc) base[sm.mdst[e] * rs] = v[e];
             }
         }
         __syncthreads();
@@ -199,33 +261,53 @@ __device__ int lu_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
                 if (i < kb) base[i * stride] = u[i];
         }
         __syncthreads();
-        // trailing update A22 -= L21 * U12 on MFMA tiles
+        // trailing update A22 -= L21 * U12 on MFMA tiles: a wave takes a strip
+        // of TSTRIP tiles along a tile row and issues all of the strip's
+        // global loads (C tiles and U12 fragments) before its MFMAs
         const int mt = m - kb;  // rows / cols of A22
         if (mt > 0) {
             using Acc = typename BMfma<T>::acc;
-            const int tiles = (mt + 15) / 16;
+            constexpr int KS = NB / 4;
+            const int tiles = (mt + 15) / 16, strips = (tiles + TSTRIP - 1) / TSTRIP;
             const int lr = lane & 15, lk = lane >> 4;
-            for (int t = wave; t < tiles * tiles; t += BTH / 64) {
-                const int ti = t / tiles, tj = t - ti * tiles;
-                const int gj = tj * 16 + lr;  // column within A22
-                Acc acc;
+            T *C = A + (long)(k + kb) * n + k + kb;
+            const T *U = A + (long)k * n + k + kb;
+            for (int t = wave; t < tiles * strips; t += BNW) {
+                const int ti = t / strips, tj0 = (t - ti * strips) * TSTRIP;
+                Acc acc[TSTRIP];
+                T bf[TSTRIP][KS];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int gi = ti * 16 + BMfma<T>::row(lk, r);
-                    acc[r] = (gi < mt && gj < mt) ? A[(long)(k + kb + gi) * n + k + kb + gj] : T(0);
+                for (int s = 0; s < TSTRIP; ++s) {
+                    const int gj = (tj0 + s) * 16 + lr;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int gi = ti * 16 + BMfma<T>::row(lk, r);
+                        acc[s][r] = (gi < mt && gj < mt) ? C[(long)gi * n + gj] : T(0);
+                    }
+#pragma unroll
+                    for (int q = 0; q < KS; ++q) {
+                        const int kk = 4 * q + lk;
+                        bf[s][q] = (gj < mt && kk < kb) ? U[(long)kk * n + gj] : T(0);
+                    }
+                }
+                T af[KS];
+#pragma unroll
+                for (int q = 0; q < KS; ++q) {
+                    const int kk = 4 * q + lk, ai = ti * 16 + lr;
+                    af[q] = (ai < mt && kk < kb) ? -P[(kb + ai) * LD + kk] : T(0);
                 }
 #pragma unroll
-                for (int ks = 0; ks < NB; ks += 4) {
-                    const int kk = ks + lk;
-                    const int ai = ti * 16 + lr;
-                    const T af = (ai < mt && kk < kb) ? -P[(kb + ai) * LD + kk] : T(0);
-                    const T bf = (gj < mt && kk < kb) ? A[(long)(k + kk) * n + k + kb + gj] : T(0);
-                    acc = BMfma<T>::op(af, bf, acc);
-                }
+                for (int s = 0; s < TSTRIP; ++s)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int gi = ti * 16 + BMfma<T>::row(lk, r);
-                    if (gi < mt && gj < mt) A[(long)(k + kb + gi) * n + k + kb + gj] = acc[r];
+                    for (int q = 0; q < KS; ++q) acc[s] = BMfma<T>::op(af[q], bf[s][q], acc[s]);
+#pragma unroll
+                for (int s = 0; s < TSTRIP; ++s) {
+                    const int gj = (tj0 + s) * 16 + lr;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int gi = ti * 16 + BMfma<T>::row(lk, r);
+                        if (gi < mt && gj < mt) C[(long)gi * n + gj] = acc[s][r];
+                    }
                 }
             }
             // RHS rows below the block: B[r][k+kb+i] -= sum_c L21[i][c] * Y1[r][c]
@@ -248,77 +330,95 @@ enum { TS_LOWER_UNIT = 0, TS_LOWER = 1, TS_UPPER = 2, TS_LOWER_T = 3 };
 
 // In-place triangular solve on nrhs vectors X (vector r at X + r*n) with the
 // triangle of A named by MODE, blocked by 32 rows.  The diagonal block and the
-// block of X are staged in LDS and swept column by column; the remaining
-// rows are updated from contiguous row segments of A.  The sweeps keep the
-// reference's orders inside a block: back_substitution / forward_substitution
-// divide by the stored diagonal, transpose_back_substitution divides and then
-// subtracts x[i] * l[i][j] (cholesky.rs:329-361).
+// block of X are staged in LDS; each wave sweeps two vectors of the block in
+// registers (half-wave per vector, lane i holding x[i], x[c] broadcast by a
+// lane read), so the sweep has no workgroup barrier per column; then the
+// remaining rows are updated from contiguous row segments of A.  The sweeps
+// keep the reference's orders inside a block: back_substitution /
+// forward_substitution divide by the stored diagonal,
+// transpose_back_substitution divides and then subtracts x[i] * l[i][j]
+// (cholesky.rs:329-361).
 template <class T, int MODE>
 __device__ void tri_solve(const T *__restrict__ A, int n, T *__restrict__ X, int nrhs, Smem &sm) {
     constexpr int TB = 32, DL = TB + 1;
     T *D = reinterpret_cast<T *>(sm.panel);
     T *XB = D + TB * DL;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool fwd = MODE == TS_LOWER_UNIT || MODE == TS_LOWER;
     const int nblk = (n + TB - 1) / TB;
     for (int bi = 0; bi < nblk; ++bi) {
         const int blk = fwd ? bi : nblk - 1 - bi;
         const int b0 = blk * TB, bs = min(TB, n - b0);
-        for (int e = tid; e < bs * bs; e += BTH) {
-            const int r = e / bs, c = e - r * bs;
-            D[r * DL + c] = A[(long)(b0 + r) * n + b0 + c];
-        }
-        for (int e = tid; e < nrhs * bs; e += BTH) {
-            const int r = e / bs, i = e - r * bs;
-            XB[r * TB + i] = X[(long)r * n + b0 + i];
+        {   // stage the diagonal block and the block of X (all loads first)
+            constexpr int PER = TB * TB / BTH;
+            T v[PER];
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int e = tid + q * BTH, r = e / TB, c = e % TB;
+                v[q] = (r < bs && c < bs) ? A[(long)(b0 + r) * n + b0 + c] : T(0);
+            }
+            T xv[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int e = tid + q * BTH, r = e / TB, i = e % TB;
+                xv[q] = (r < nrhs && i < bs) ? X[(long)r * n + b0 + i] : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int e = tid + q * BTH;
+                D[(e / TB) * DL + e % TB] = v[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int e = tid + q * BTH;
+                if (e / TB < nrhs) XB[e] = xv[q];
+            }
         }
         __syncthreads();
         {
 #pragma clang fp contract(off)
-            if (fwd) {
-                for (int c = 0; c < bs; ++c) {
-                    if (MODE == TS_LOWER) {
-                        if (tid < nrhs) XB[tid * TB + c] = XB[tid * TB + c] / D[c * DL + c];
-                        __syncthreads();
+            const int i = lane & 31, half = lane >> 5;
+            for (int r0 = 2 * wave; r0 < nrhs; r0 += 2 * BNW) {
+                const int r = r0 + half;
+                T x = r < nrhs ? XB[r * TB + i] : T(0);
+                if (fwd) {
+                    for (int c = 0; c < bs; ++c) {
+                        const T l = D[i * DL + c];
+                        if (MODE == TS_LOWER && i == c) x = x / D[c * DL + c];
+                        const T lo = lane_get(x, c), hi = lane_get(x, 32 + c);
+                        const T xc = half ? hi : lo;
+                        if (i > c) x = x - l * xc;
                     }
-                    for (int q = tid; q < nrhs * bs; q += BTH) {
-                        const int r = q / bs, i = q - r * bs;
-                        if (i > c) XB[r * TB + i] = XB[r * TB + i] - D[i * DL + c] * XB[r * TB + c];
+                } else {
+                    for (int c = bs - 1; c >= 0; --c) {
+                        const T l = MODE == TS_UPPER ? D[i * DL + c] : D[c * DL + i];
+                        if (i == c) x = x / D[c * DL + c];
+                        const T lo = lane_get(x, c), hi = lane_get(x, 32 + c);
+                        const T xc = half ? hi : lo;
+                        if (i < c) x = x - (MODE == TS_UPPER ? l * xc : xc * l);
                     }
-                    __syncthreads();
                 }
-            } else {
-                for (int c = bs - 1; c >= 0; --c) {
-                    if (tid < nrhs) XB[tid * TB + c] = XB[tid * TB + c] / D[c * DL + c];
-                    __syncthreads();
-                    for (int q = tid; q < nrhs * bs; q += BTH) {
-                        const int r = q / bs, i = q - r * bs;
-                        if (i < c) {
-                            const T l = MODE == TS_UPPER ? D[i * DL + c] : D[c * DL + i];
-                            XB[r * TB + i] = XB[r * TB + i] - (MODE == TS_UPPER ? l * XB[r * TB + c]
-                                                                                : XB[r * TB + c] * l);
-                        }
-                    }
-                    __syncthreads();
-                }
+                if (r < nrhs && i < bs) XB[r * TB + i] = x;
             }
         }
+        __syncthreads();
         for (int e = tid; e < nrhs * bs; e += BTH) {
             const int r = e / bs, i = e - r * bs;
             X[(long)r * n + b0 + i] = XB[r * TB + i];
         }
-        // the rest of the vector
+        // the rest of the vector (the block's entries of A loaded first)
         const int lo = fwd ? b0 + bs : 0, hi = fwd ? n : b0;
         const int cnt = hi - lo;
         for (int q = tid; q < cnt * nrhs; q += BTH) {
             const int r = q / cnt, i = lo + (q - r * cnt);
+            T av[TB];
+#pragma unroll
+            for (int c = 0; c < TB; ++c)
+                if (c < bs) av[c] = MODE == TS_LOWER_T ? A[(long)(b0 + c) * n + i] : A[(long)i * n + b0 + c];
             T s = X[(long)r * n + i];
-            if (MODE == TS_LOWER_T) {
-                for (int c = 0; c < bs; ++c) s = s - A[(long)(b0 + c) * n + i] * XB[r * TB + c];
-            } else {
-                const T *ar = A + (long)i * n + b0;
-                for (int c = 0; c < bs; ++c) s = s - ar[c] * XB[r * TB + c];
-            }
+#pragma unroll
+            for (int c = 0; c < TB; ++c)
+                if (c < bs) s = s - av[c] * XB[r * TB + c];
             X[(long)r * n + i] = s;
         }
         __syncthreads();
@@ -338,10 +438,7 @@ __device__ int chol_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
     __syncthreads();
     for (int k = 0; k < n; k += NB) {
         const int kb = min(NB, n - k), m = n - k;
-        for (int e = tid; e < m * kb; e += BTH) {
-            const int r = e / kb, c = e - r * kb;
-            if (r >= kb || c <= r) P[r * LD + c] = A[(long)(k + r) * n + k + c];
-        }
+        panel_move<T, LD, false, true>(A, n, k, m, kb, P);
         __syncthreads();
         {
 #pragma clang fp contract(off)
@@ -383,10 +480,7 @@ __device__ int chol_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
                 __syncthreads();
             }
         }
-        for (int e = tid; e < m * kb; e += BTH) {
-            const int r = e / kb, c = e - r * kb;
-            if (r >= kb || c <= r) A[(long)(k + r) * n + k + c] = P[r * LD + c];
-        }
+        panel_move<T, LD, true, true>(A, n, k, m, kb, P);
         // RHS: y1 = L11^-1 b1 (forward_substitution order per vector), then
         // b2 -= L21 y1
         for (int r = tid; r < nrhs; r += BTH) {
@@ -408,36 +502,53 @@ __device__ int chol_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
                 for (int c = 0; c < kb; ++c) s = s - P[(kb + i) * LD + c] * br[k + c];
                 br[k + kb + i] = s;
             }
-            // SYRK on lower tiles: A22 -= L21 L21^T
+            // SYRK on lower tiles: A22 -= L21 L21^T, in strips of TSTRIP tiles
+            // along a tile row (tiles right of the diagonal tile skipped), the
+            // strip's C tiles loaded before its MFMAs
             using Acc = typename BMfma<T>::acc;
+            constexpr int KS = NB / 4;
             const int tiles = (mt + 15) / 16;
             const int lr = lane & 15, lk = lane >> 4;
-            const int ntri = tiles * (tiles + 1) / 2;
-            for (int t = wave; t < ntri; t += BTH / 64) {
-                // t -> (ti, tj) with tj <= ti
-                int ti = (int)((sqrtf(8.f * t + 1.f) - 1.f) * 0.5f);
-                while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-                while (ti * (ti + 1) / 2 > t) --ti;
-                const int tj = t - ti * (ti + 1) / 2;
-                const int gj = tj * 16 + lr;
-                Acc acc;
+            T *C = A + (long)(k + kb) * n + k + kb;
+            int t = 0;
+            for (int ti = 0; ti < tiles; ++ti) {
+                T af[KS];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int gi = ti * 16 + BMfma<T>::row(lk, r);
-                    acc[r] = (gi < mt && gj < mt) ? A[(long)(k + kb + gi) * n + k + kb + gj] : T(0);
+                for (int q = 0; q < KS; ++q) {
+                    const int kk = 4 * q + lk, ai = ti * 16 + lr;
+                    af[q] = (ai < mt && kk < kb) ? -P[(kb + ai) * LD + kk] : T(0);
                 }
+                for (int tj0 = 0; tj0 <= ti; tj0 += TSTRIP, ++t) {
+                    if (t % BNW != wave) continue;
+                    Acc acc[TSTRIP];
 #pragma unroll
-                for (int ks = 0; ks < NB; ks += 4) {
-                    const int kk = ks + lk;
-                    const int ai = ti * 16 + lr;
-                    const T af = (ai < mt && kk < kb) ? -P[(kb + ai) * LD + kk] : T(0);
-                    const T bf = (gj < mt && kk < kb) ? P[(kb + gj) * LD + kk] : T(0);
-                    acc = BMfma<T>::op(af, bf, acc);
-                }
+                    for (int s = 0; s < TSTRIP; ++s) {
+                        const int gj = (tj0 + s) * 16 + lr;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int gi = ti * 16 + BMfma<T>::row(lk, r);
-                    if (gi < mt && gj < mt && gj <= gi) A[(long)(k + kb + gi) * n + k + kb + gj] = acc[r];
+                        for (int r = 0; r < 4; ++r) {
+                            const int gi = ti * 16 + BMfma<T>::row(lk, r);
+                            acc[s][r] = (tj0 + s <= ti && gi < mt && gj < mt) ? C[(long)gi * n + gj] : T(0);
+                        }
+                    }
+#pragma unroll
+                    for (int s = 0; s < TSTRIP; ++s) {
+                        const int gj = (tj0 + s) * 16 + lr;
+#pragma unroll
+                        for (int q = 0; q < KS; ++q) {
+                            const int kk = 4 * q + lk;
+                            const T bf = (gj < mt && kk < kb) ? P[(kb + gj) * LD + kk] : T(0);
+                            acc[s] = BMfma<T>::op(af[q], bf, acc[s]);
+                        }
+                    }
+#pragma unroll
+                    for (int s = 0; s < TSTRIP; ++s) {
+                        const int gj = (tj0 + s) * 16 + lr;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int gi = ti * 16 + BMfma<T>::row(lk, r);
+                            if (tj0 + s <= ti && gi < mt && gj < mt && gj <= gi) C[(long)gi * n + gj] = acc[s][r];
+                        }
+                    }
                 }
             }
         }
@@ -469,7 +580,7 @@ __global__ void k_fill_u32_graph(unsigned *p, unsigned count) {
     if (threadIdx.x < count) p[threadIdx.x] = 0u;
 }
 
-__global__ __launch_bounds__(BTH) void k_batch_f64(rlg_batch_item *__restrict__ items,
+__global__ __launch_bounds__(BTH, 2) void k_batch_f64(rlg_batch_item *__restrict__ items,
                                                    const int *__restrict__ order,
                                                    const unsigned char *__restrict__ nbs, int count,
                                                    unsigned *__restrict__ counter) {
