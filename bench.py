@@ -411,8 +411,14 @@ def batched(lib, torch, dev, count=20000, nmin=8, nmax=512, seed=30):
 
     out = {"systems": count, "n_range": [nmin, nmax], "rhs_range": [1, 8],
            "flop_per_batch": work}
+    # panel width per n-bucket from the library's autotuner (untimed)
+    rep = C.create_string_buffer(4096)
+    t0 = time.time()
+    assert lib.rlg_batch_autotune(64, rep, len(rep)) == 0, _lib.last_error()
+    out["autotune"] = [line.split(";")[0] for line in rep.value.decode().splitlines()]
+    log(f"batched: autotuned in {time.time() - t0:.1f}s: {out['autotune']}")
     for mixed in (False, True):
-        run(mixed)  # warm (and the autotuner's one-time sweep, if any)
+        run(mixed)  # warm
         times = []
         for _ in range(3):
             ms, fb = run(mixed)

@@ -70,7 +70,8 @@ struct Smem {
     int red_i[BTH / 64];
     int piv[32];
     double div[32];
-    int mdst[64], msrc[64];  // net row moves of a panel's swaps
+    double prow[32], jrow[32];  // pivot row and row j of a panel column step
+    int mdst[64], msrc[64];     // net row moves of a panel's swaps
     int mcnt;
     int info;
     int item;
@@ -143,15 +144,6 @@ __device__ __forceinline__ void panel_moves(int kb, const int *piv, int *mdst, i
     }
 }
 
-template <class T>
-__device__ __forceinline__ bool bbetter(T v2, int i2, T v1, int i1) {
-    if (i2 < 0) return false;
-    if (i1 < 0) return true;
-    if (v2 > v1) return true;
-    if (v1 > v2) return false;
-    return i2 < i1;
-}
-
 // ---------------------------------------------------------------- LU
 
 // Factor the n x n matrix A (ld = n, type T) in place with RHS vectors B
@@ -170,49 +162,92 @@ __device__ int lu_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
         const int kb = min(NB, n - k), m = n - k;
         panel_move<T, LD, false, false>(A, n, k, m, kb, P);
         __syncthreads();
-        for (int j = 0; j < kb; ++j) {
+        // Column j of the panel, two barriers: (1) the waves' candidates are
+        // in LDS; every thread picks the same pivot row p and reads what it
+        // needs of rows j and p and of its own rows; (2) the rows are
+        // rewritten (swap, multipliers, update of the columns right of j) and
+        // each thread searches column j + 1 over its own rows, which only it
+        // writes.  Thread t owns the panel rows t + q BTH.
+        constexpr int RQ = NB == 32 ? 1 : 2;  // NB = 32 only for m <= 247
+        {
             T bv = T(0);
-            int bi = -1;
-            for (int r = j + tid; r < m; r += BTH) {
-                const T v = fabs(P[r * LD + j]);
-                if (bbetter(v, r, bv, bi)) { bv = v; bi = r; }
-            }
+            int bi = INT_MAX;
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const T ov = __shfl_xor(bv, off, 64);
-                const int oi = __shfl_xor(bi, off, 64);
-                if (bbetter(ov, oi, bv, bi)) { bv = ov; bi = oi; }
+            for (int q = 0; q < RQ; ++q) {
+                const int r = tid + q * BTH;
+                const T v = r < m ? fabs(P[r * LD]) : T(0);
+                const bool take = (r < m) & better(v, r, bv, bi);
+                bv = take ? v : bv;
+                bi = take ? r : bi;
             }
+            const int wl = wave_first_max_lane(bv, bi);
+            bv = wl >= 0 ? lane_get(bv, wl) : T(0);
+            bi = wl >= 0 ? lane_get(bi, wl) : INT_MAX;
             if (lane == 0) { sm.red_v[wave] = (double)bv; sm.red_i[wave] = bi; }
-            __syncthreads();
-            if (tid == 0) {
-                T v = (T)sm.red_v[0];
-                int i = sm.red_i[0];
-                for (int q = 1; q < BTH / 64; ++q)
-                    if (bbetter((T)sm.red_v[q], sm.red_i[q], v, i)) { v = (T)sm.red_v[q]; i = sm.red_i[q]; }
-                sm.piv[j] = i;
-                piv_out[k + j] = k + i;
-                if (fabs(P[i * LD + j]) < beps<T>() && sm.info == n) sm.info = k + j;
-            }
-            __syncthreads();
-            const int p = sm.piv[j];
-            if (p != j && tid < kb) {
-                const T t = P[j * LD + tid];
-                P[j * LD + tid] = P[p * LD + tid];
-                P[p * LD + tid] = t;
-            }
-            __syncthreads();
-            {
+        }
+        __syncthreads();
+        T *prow = reinterpret_cast<T *>(sm.prow), *jrow = reinterpret_cast<T *>(sm.jrow);
+        for (int j = 0; j < kb; ++j) {
 #pragma clang fp contract(off)
-                const T piv = P[j * LD + j];
-                for (int r = j + 1 + tid; r < m; r += BTH) {
-                    const T mult = P[r * LD + j] / piv;
+            T pa = (T)sm.red_v[0];
+            int p = sm.red_i[0];
+#pragma unroll
+            for (int q = 1; q < BNW; ++q) {
+                const bool take = better((T)sm.red_v[q], sm.red_i[q], pa, p);
+                pa = take ? (T)sm.red_v[q] : pa;
+                p = take ? sm.red_i[q] : p;
+            }
+            p = __builtin_amdgcn_readfirstlane(p);  // the same in every lane
+            const T piv = P[p * LD + j];
+            // rows p and j set aside (the pivot row with zeros at columns <= j:
+            // the sweep below then leaves those columns as they are)
+            if (tid < NB) {
+                prow[tid] = (tid < kb) ? P[p * LD + tid] : T(0);
+                jrow[tid] = (tid < kb) ? P[j * LD + tid] : T(0);
+            }
+            if (tid == 0) {
+                sm.piv[j] = p;
+                if (pa < beps<T>() && sm.info == n) sm.info = k + j;
+            }
+            __syncthreads();
+            if (tid < kb) {
+                P[j * LD + tid] = prow[tid];
+                if (tid < j) P[p * LD + tid] = jrow[tid];
+            }
+            T bv = T(0);
+            int bi = INT_MAX;
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int r = tid + q * BTH;
+                if (r < m && r > j) {
+                    const T *src = r == p ? jrow : P + r * LD;  // row r after the swap
+                    const T mult = src[j] / piv;
+                    T x[NB], pm[NB];
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) {
+                        x[c] = src[c];
+                        pm[c] = c > j ? prow[c] : T(0);
+                    }
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) P[r * LD + c] = x[c] - mult * pm[c];
                     P[r * LD + j] = mult;
-                    for (int c = j + 1; c < kb; ++c) P[r * LD + c] = P[r * LD + c] - mult * P[j * LD + c];
+                    if (j + 1 < kb) {
+                        const T v = fabs(P[r * LD + j + 1]);
+                        const bool take = better(v, r, bv, bi);
+                        bv = take ? v : bv;
+                        bi = take ? r : bi;
+                    }
                 }
+            }
+            if (j + 1 < kb) {
+                const int wl = wave_first_max_lane(bv, bi);
+                bv = wl >= 0 ? lane_get(bv, wl) : T(0);
+                bi = wl >= 0 ? lane_get(bi, wl) : INT_MAX;
+                if (lane == 0) { sm.red_v[wave] = (double)bv; sm.red_i[wave] = bi; }
             }
             __syncthreads();
         }
+        if (tid < kb) piv_out[k + tid] = k + sm.piv[tid];
         panel_move<T, LD, true, false>(A, n, k, m, kb, P);
         // row swaps on the columns outside the panel and on the RHS, as the
         // net moves of the panel's swaps: every source read before any write
@@ -220,7 +255,7 @@ __device__ int lu_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
         __syncthreads();
         const int right = n - k - kb;
         const int ncols = k + right + nrhs;
-        const int mc = sm.mcnt;
+        const int mc = __builtin_amdgcn_readfirstlane(sm.mcnt);
         if (mc > 0) {
             for (int q = tid; q < ncols; q += BTH) {
                 T *base;
@@ -250,11 +285,17 @@ __device__ int lu_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
             for (int i = 0; i < NB; ++i) u[i] = i < kb ? base[i * stride] : T(0);
             {
 #pragma clang fp contract(off)
+                // unguarded: in a last panel narrower than NB the entries
+                // u[kb..] take garbage from LDS rows past the panel and are
+                // never stored
 #pragma unroll
-                for (int c = 0; c < NB; ++c)
+                for (int c = 0; c < NB; ++c) {
+                    // one column of L11 at a time: scheduled freely, the
+                    // unrolled sweep would hold all of L11 in registers
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int i = c + 1; i < NB; ++i)
-                        if (i < kb) u[i] = u[i] - P[i * LD + c] * u[c];
+                    for (int i = c + 1; i < NB; ++i) u[i] = u[i] - P[i * LD + c] * u[c];
+                }
             }
 #pragma unroll
             for (int i = 1; i < NB; ++i)
@@ -468,14 +509,14 @@ __device__ int chol_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
                         P[r * LD + j] = P[r * LD + j] - s;
                     }
                 }
+                // the diagonal entry (row j is thread 0's) kept apart: the
+                // division below rewrites it while other threads read it
+                if (tid == 0) sm.div[j] = (double)P[j * LD + j];
                 __syncthreads();
-                const T d = P[j * LD + j];
-                if (tid == 0) {
-                    if ((fabs(d) < beps<T>() || d < T(0)) && sm.info == n) sm.info = k + j;
-                    sm.div[j] = (double)sqrt(d);
-                }
-                __syncthreads();
-                const T dv = (T)sm.div[j];
+                // every thread takes the square root of the diagonal itself
+                const T d = (T)sm.div[j];
+                const T dv = sqrt(d);
+                if (tid == 0 && (fabs(d) < beps<T>() || d < T(0)) && sm.info == n) sm.info = k + j;
                 for (int r = j + tid; r < m; r += BTH) P[r * LD + j] = P[r * LD + j] / dv;
                 __syncthreads();
             }
@@ -588,7 +629,7 @@ __global__ __launch_bounds__(BTH, 2) void k_batch_f64(rlg_batch_item *__restrict
     for (;;) {
         if (threadIdx.x == 0) sm.item = (int)atomicAdd(counter, 1u);
         __syncthreads();
-        const int idx = sm.item;
+        const int idx = __builtin_amdgcn_readfirstlane(sm.item);
         __syncthreads();
         if (idx >= count) break;
         const int id = order[idx];
@@ -656,7 +697,7 @@ __global__ __launch_bounds__(BTH) void k_batch_mixed(rlg_batch_item *__restrict_
     for (;;) {
         if (tid == 0) sm.item = (int)atomicAdd(counter, 1u);
         __syncthreads();
-        const int idx = sm.item;
+        const int idx = __builtin_amdgcn_readfirstlane(sm.item);
         __syncthreads();
         if (idx >= count) break;
         const int id = order[idx];

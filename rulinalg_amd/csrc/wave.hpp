@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <type_traits>
 
 namespace rlg {
@@ -78,6 +79,45 @@ __device__ __forceinline__ int wave_min_i32(int k) {
     k = min(k, dpp<DPP_HALF_MIRROR>(k));
     k = min(k, dpp<DPP_MIRROR>(k));
     return min(min(lane_get(k, 0), lane_get(k, 16)), min(lane_get(k, 32), lane_get(k, 48)));
+}
+
+// ---------------------------------------------------------------- pivot search
+
+// Candidate order: larger magnitude first; equal magnitudes go to the lower
+// row index, which reproduces the reference's first-maximum scan.
+template <class T>
+__device__ __forceinline__ bool better(T v2, int i2, T v1, int i1) {
+    // branch-free: the exec-mask branches of an if-chain cost more than the tests
+    return (i2 != INT_MAX) & ((i1 == INT_MAX) | (v2 > v1) | (!(v1 > v2) & (i2 < i1)));
+}
+
+// The lane that holds the wave's first maximum under `better` (largest
+// |value|, ties to the smallest row index; i == INT_MAX marks a lane without
+// a candidate), or -1 if no lane has one; uniform.  One 64-bit maximum over
+// the wave: in FP32 the key packs the value's bits (a non-negative float
+// orders like its bits) above the complemented row index, so keys are
+// distinct; in FP64 the key is the value's bits + 1 and a tie is settled by
+// the smallest index among the tied lanes.  A NaN ranks like zero.
+__device__ __forceinline__ int wave_first_max_lane(float v, int i) {
+    unsigned long long key = 0;
+    if (i != INT_MAX) {
+        const unsigned b = (v == v) ? __float_as_uint(v) : 0u;
+        key = ((unsigned long long)b << 32) | (0xFFFFFFFFu - (unsigned)i);
+    }
+    const unsigned long long m = wave_max_u64(key);
+    if (m == 0) return -1;
+    return __ffsll((long long)__ballot(key == m)) - 1;
+}
+__device__ __forceinline__ int wave_first_max_lane(double v, int i) {
+    unsigned long long key = 0;
+    if (i != INT_MAX) key = (v == v) ? (unsigned long long)__double_as_longlong(v) + 1 : 1;
+    const unsigned long long m = wave_max_u64(key);
+    if (m == 0) return -1;
+    const unsigned long long tied = __ballot(key == m);
+    if (__popcll(tied) == 1) return __ffsll((long long)tied) - 1;
+    const int ii = key == m ? i : INT_MAX;
+    const int mi = wave_min_i32(ii);
+    return __ffsll((long long)__ballot(ii == mi)) - 1;
 }
 
 }  // namespace rlg
