@@ -669,20 +669,17 @@ __device__ MixedSlot slot_of(char *ws, int s) {
     return m;
 }
 
-// max over i of |v[i]| (all threads return the same value)
-__device__ double block_maxabs_sum(double v, Smem &sm, bool is_max) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_xor(v, off, 64);
-        v = is_max ? fmax(v, o) : v + o;
-    }
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) sm.red_v[threadIdx.x >> 6] = v;
-    __syncthreads();
-    double r = sm.red_v[0];
-    for (int q = 1; q < BTH / 64; ++q) r = is_max ? fmax(r, sm.red_v[q]) : r + sm.red_v[q];
-    __syncthreads();
-    return r;
+// Sum and maximum over the wave (uniform results, fixed order).
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    v = row16_sum(v);
+    return (lane_get(v, 0) + lane_get(v, 16)) + (lane_get(v, 32) + lane_get(v, 48));
+}
+__device__ __forceinline__ double wave_max_f64(double v) {
+    v = fmax(v, dpp<DPP_XOR1>(v));
+    v = fmax(v, dpp<DPP_XOR2>(v));
+    v = fmax(v, dpp<DPP_HALF_MIRROR>(v));
+    v = fmax(v, dpp<DPP_MIRROR>(v));
+    return fmax(fmax(lane_get(v, 0), lane_get(v, 16)), fmax(lane_get(v, 32), lane_get(v, 48)));
 }
 
 __global__ __launch_bounds__(BTH) void k_batch_mixed(rlg_batch_item *__restrict__ items,
@@ -704,15 +701,33 @@ __global__ __launch_bounds__(BTH) void k_batch_mixed(rlg_batch_item *__restrict_
         rlg_batch_item it = items[id];
         const int n = it.n, nrhs = it.nrhs;
         const double *A = it.a;
-        // FP32 copy, ||A||inf
-        double rs = 0.0, anorm = 0.0;
-        for (int e = tid; e < n * n; e += BTH) ms.a32[e] = (float)A[e];
-        for (int i = tid; i < n; i += BTH) {
-            double s = 0.0;
-            for (int j = 0; j < n; ++j) s += fabs(A[(long)i * n + j]);
-            rs = fmax(rs, s);
+        const int lane = tid & 63, wave = tid >> 6;
+        // per-vector norms and convergence state in the panel buffer past
+        // what the triangular solves (first 1568 doubles) and the row map use
+        double *bmax = sm.panel + 6144, *xmax = bmax + BMAXR, *rmax = xmax + BMAXR;  // rmax[BNW][BMAXR]
+        double *etab = rmax + BNW * BMAXR;                                            // etab[2][BMAXR]
+        // FP32 copy (eight loads in flight), ||A||inf one row per wave
+        for (int e0 = tid; e0 < n * n; e0 += 8 * BTH) {
+            double v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = e0 + q * BTH < n * n ? A[e0 + q * BTH] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (e0 + q * BTH < n * n) ms.a32[e0 + q * BTH] = (float)v[q];
         }
-        anorm = block_maxabs_sum(rs, sm, true);
+        {
+            double rs = 0.0;
+            for (int i = wave; i < n; i += BNW) {
+                double s = 0.0;
+                for (int j = lane; j < n; j += 64) s += fabs(A[(long)i * n + j]);
+                rs = fmax(rs, wave_sum_f64(s));
+            }
+            if (lane == 0) sm.red_v[wave] = rs;
+        }
+        __syncthreads();
+        double anorm = sm.red_v[0];
+#pragma unroll
+        for (int q = 1; q < BNW; ++q) anorm = fmax(anorm, sm.red_v[q]);
         __syncthreads();
         int info;
         if (it.kind == RLG_KIND_LU) {
@@ -735,52 +750,101 @@ __global__ __launch_bounds__(BTH) void k_batch_mixed(rlg_batch_item *__restrict_
                 if (p != k) { const int t = raw[k]; raw[k] = raw[p]; raw[p] = t; }
             }
         }
-        __syncthreads();
-        bool converged = false;
-        if (info == n) {
-            for (int r = 0; r < nrhs; ++r) {
-                double *x = ms.x + (long)r * n;
-                double *res = ms.res + (long)r * n;
-                float *d = ms.d32 + (long)r * n;
-                const double *b = it.b + (long)r * n;
-                double bn = 0.0;
-                for (int i = tid; i < n; i += BTH) { x[i] = 0.0; res[i] = b[i]; bn = fmax(bn, fabs(b[i])); }
-                const double bnorm = block_maxabs_sum(bn, sm, true);
-                double prev = INFINITY, eta = INFINITY;
-                for (int itn = 0; itn < max_iter; ++itn) {
-                    if (it.kind == RLG_KIND_LU) {
-                        for (int i = tid; i < n; i += BTH) d[i] = (float)res[raw[i]];
-                        __syncthreads();
-                        tri_solve<float, TS_LOWER_UNIT>(ms.a32, n, d, 1, sm);
-                        tri_solve<float, TS_UPPER>(ms.a32, n, d, 1, sm);
-                    } else {
-                        for (int i = tid; i < n; i += BTH) d[i] = (float)res[i];
-                        __syncthreads();
-                        tri_solve<float, TS_LOWER>(ms.a32, n, d, 1, sm);
-                        tri_solve<float, TS_LOWER_T>(ms.a32, n, d, 1, sm);
-                    }
-                    double xn = 0.0;
-                    for (int i = tid; i < n; i += BTH) { x[i] += (double)d[i]; xn = fmax(xn, fabs(x[i])); }
-                    const double xnorm = block_maxabs_sum(xn, sm, true);
-                    double rn = 0.0;
-                    for (int i = tid; i < n; i += BTH) {
-                        const double *ar = A + (long)i * n;
-                        double s = 0.0;
-                        for (int j = 0; j < n; ++j) s += ar[j] * x[j];
-                        res[i] = b[i] - s;
-                        rn = fmax(rn, fabs(res[i]));
-                    }
-                    const double rnorm = block_maxabs_sum(rn, sm, true);
-                    const double den = anorm * xnorm + bnorm;
-                    eta = den > 0 ? rnorm / den : 0.0;
-                    if (eta <= tol) break;
-                    if (itn >= 1 && !(eta <= 0.5 * prev)) break;
-                    prev = eta;
-                }
-                converged = eta <= tol;
-                if (!converged) break;
+        // refinement of all right-hand sides together: x = 0, r = b, |b|
+        double *X = ms.x, *R = ms.res;
+        float *D = ms.d32;
+        for (int r = wave; r < nrhs; r += BNW) {
+            double bm = 0.0;
+            for (int i = lane; i < n; i += 64) {
+                const double bv = it.b[(long)r * n + i];
+                X[(long)r * n + i] = 0.0;
+                R[(long)r * n + i] = bv;
+                bm = fmax(bm, fabs(bv));
             }
+            bm = wave_max_f64(bm);
+            if (lane == 0) bmax[r] = bm;
         }
+        __syncthreads();
+        // done: vectors whose backward error reached tol (the same bits in
+        // every thread); a vector that stops halving its error fails the system
+        unsigned done = 0u;
+        const unsigned all = nrhs >= 32 ? ~0u : (1u << nrhs) - 1u;
+        bool stalled = info != n;
+        for (int itn = 0; itn < max_iter && !stalled && done != all; ++itn) {
+            for (int e = tid; e < nrhs * n; e += BTH) {
+                const int r = e / n, i = e - r * n;
+                const int src = it.kind == RLG_KIND_LU ? raw[i] : i;
+                D[e] = (done >> r) & 1u ? 0.f : (float)R[(long)r * n + src];
+            }
+            __syncthreads();
+            if (it.kind == RLG_KIND_LU) {
+                tri_solve<float, TS_LOWER_UNIT>(ms.a32, n, D, nrhs, sm);
+                tri_solve<float, TS_UPPER>(ms.a32, n, D, nrhs, sm);
+            } else {
+                tri_solve<float, TS_LOWER>(ms.a32, n, D, nrhs, sm);
+                tri_solve<float, TS_LOWER_T>(ms.a32, n, D, nrhs, sm);
+            }
+            // x += d and |x|, one vector per wave
+            for (int r = wave; r < nrhs; r += BNW) {
+                if ((done >> r) & 1u) continue;
+                double xm = 0.0;
+                for (int i = lane; i < n; i += 64) {
+                    const double v = X[(long)r * n + i] + (double)D[(long)r * n + i];
+                    X[(long)r * n + i] = v;
+                    xm = fmax(xm, fabs(v));
+                }
+                xm = wave_max_f64(xm);
+                if (lane == 0) xmax[r] = xm;
+            }
+            __syncthreads();
+            // r = b - A x in FP64, one row per wave, four vectors per pass
+            // over the row; |r| per vector kept in the wave's own LDS slots
+            if (lane == 0) {
+#pragma unroll
+                for (int r = 0; r < BMAXR; ++r) rmax[wave * BMAXR + r] = 0.0;
+            }
+            for (int i = wave; i < n; i += BNW) {
+                for (int rb = 0; rb < nrhs; rb += 4) {
+                    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+                    for (int j = lane; j < n; j += 64) {
+                        const double a = A[(long)i * n + j];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (rb + q < nrhs) acc[q] += a * X[(long)(rb + q) * n + j];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (rb + q < nrhs) {
+                            const int r = rb + q;
+                            const double res = it.b[(long)r * n + i] - wave_sum_f64(acc[q]);
+                            if (lane == 0) {
+                                R[(long)r * n + i] = res;
+                                rmax[wave * BMAXR + r] = fmax(rmax[wave * BMAXR + r], fabs(res));
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            // every thread takes the same decisions; the errors of this step
+            // go to etab[itn & 1], the previous step's are read from the other
+            for (int r = 0; r < nrhs; ++r) {
+                if ((done >> r) & 1u) continue;
+                double rn = rmax[r];
+#pragma unroll
+                for (int q = 1; q < BNW; ++q) rn = fmax(rn, rmax[q * BMAXR + r]);
+                const double den = anorm * xmax[r] + bmax[r];
+                const double eta = den > 0 ? rn / den : 0.0;
+                if (eta <= tol) {
+                    done |= 1u << r;
+                } else if (itn >= 1 && !(eta <= 0.5 * etab[((itn + 1) & 1) * BMAXR + r])) {
+                    stalled = true;
+                }
+                if (tid == 0) etab[(itn & 1) * BMAXR + r] = eta;
+            }
+            __syncthreads();
+        }
+        const bool converged = !stalled && done == all;
         if (converged) {
             for (int e = tid; e < nrhs * n; e += BTH) it.b[e] = ms.x[e];
             if (tid == 0) items[id].status = RLG_OK;
