@@ -64,7 +64,9 @@ template <> struct StageK<float> { static constexpr int value = 2 * GBK; };
 // loaded with 16-byte loads (a quarter (FP32) or half (FP64) of the load
 // instructions), which measured 18 % faster on the LU's depth-256 FP32
 // update.  The untransposed B stage is padded by 16 bytes per row.
-template <class T, bool TRANSB, bool LOWER, bool VEC>
+// CP: the C tile is read during the last stage (FP32 with VEC only; chosen
+// for shallow products, where the epilogue is a large share of a tile's time).
+template <class T, bool TRANSB, bool LOWER, bool VEC, bool CP>
 __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
     int M, int N, int K, T alpha, const T *__restrict__ A, long lda,
     const T *__restrict__ B, long ldb, T beta, T *__restrict__ C, long ldc,
@@ -165,6 +167,27 @@ __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
         }
     };
 
+    // FP32 with 16-byte tile loads: the first two row groups of the C tile
+    // are read during the last stage's MFMAs (32 more registers, within the
+    // two-workgroups-per-CU budget; all four, or the scalar-load form, would
+    // spill), the other two before the first store.  Measured: depth-256
+    // updates 83 -> 89 TF/s; at depth 8192 it costs 6 %, hence CP.
+    constexpr bool CPRE = sizeof(T) == 4 && VEC && CP;
+    constexpr int CG = CPRE ? 2 : 1;
+    T cpre[CG][4][4];
+    auto load_c = [&]() {
+#pragma unroll
+        for (int i = 0; i < CG; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gi = m0 + wm + i * 16 + Mfma<T>::row(lk, r);
+                    const int gj = n0 + wn + j * 16 + lr;
+                    cpre[i][j][r] = (beta != T(0) && gi < M && gj < N) ? C[(long)gi * ldc + gj] : T(0);
+                }
+    };
+
     const int KT = (K + BK - 1) / BK;
     load(0);
     store(0);
@@ -172,6 +195,7 @@ __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
     for (int kt = 0; kt < KT; ++kt) {
         const int buf = kt & 1;
         if (kt + 1 < KT) load((kt + 1) * BK);
+        else if constexpr (CPRE) load_c();
 #pragma unroll
         for (int ks = 0; ks < BK / 4; ++ks) {
             T af[4], bf[4];
@@ -187,9 +211,26 @@ __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
         if (kt + 1 < KT) store(buf ^ 1);
         __syncthreads();
     }
+    if constexpr (CPRE) {
+        if (KT == 0) load_c();  // K = 0: no stage read it
+    }
 
     // epilogue in four row groups: the 16 loads of C of a group are issued
-    // before its stores (a load behind a store to C would wait for it)
+    // before its stores (a load behind a store to C would wait for it); in
+    // FP32 groups 0-1 are in cpre and groups 2-3 are loaded before any store
+    T clate[CPRE ? 2 : 1][4][4];
+    if constexpr (CPRE) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gi = m0 + wm + (i + 2) * 16 + Mfma<T>::row(lk, r);
+                    const int gj = n0 + wn + j * 16 + lr;
+                    clate[i][j][r] = (beta != T(0) && gi < M && gj < N) ? C[(long)gi * ldc + gj] : T(0);
+                }
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         T cin[4][4];
@@ -199,7 +240,8 @@ __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
             for (int r = 0; r < 4; ++r) {
                 const int gi = m0 + wm + i * 16 + Mfma<T>::row(lk, r);
                 const int gj = n0 + wn + j * 16 + lr;
-                cin[j][r] = (beta != T(0) && gi < M && gj < N) ? C[(long)gi * ldc + gj] : T(0);
+                if constexpr (CPRE) cin[j][r] = i < 2 ? cpre[i < 2 ? i : 0][j][r] : clate[i >= 2 ? i - 2 : 0][j][r];
+                else cin[j][r] = (beta != T(0) && gi < M && gj < N) ? C[(long)gi * ldc + gj] : T(0);
             }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
@@ -549,11 +591,16 @@ static int gemm_impl(hipStream_t s, const char *cls, size_t m, size_t n, size_t 
     auto al16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
     const bool vec = k % VL == 0 && lda % VL == 0 && ldb % VL == 0 && al16(a) && al16(b) &&
                      (trans_b || n % VL == 0);
-#define RLG_GEMM_LAUNCH(TB, LO, V)                                                               \
-    hipLaunchKernelGGL((k_gemm<T, TB, LO, V>), grid, block, 0, s, (int)m, (int)n, (int)k, alpha, \
+    const bool cp = vec && sizeof(T) == 4 && k <= 1024;
+#define RLG_GEMM_LAUNCH(TB, LO, V, CPF)                                                               \
+    hipLaunchKernelGGL((k_gemm<T, TB, LO, V, CPF>), grid, block, 0, s, (int)m, (int)n, (int)k, alpha, \
                        a, (long)lda, b, (long)ldb, beta, c, (long)ldc, tiles_m, tiles_n)
-#define RLG_GEMM_LAUNCH_V(TB, LO) \
-    do { if (vec) RLG_GEMM_LAUNCH(TB, LO, true); else RLG_GEMM_LAUNCH(TB, LO, false); } while (0)
+#define RLG_GEMM_LAUNCH_V(TB, LO)                                                   \
+    do {                                                                            \
+        if (cp) RLG_GEMM_LAUNCH(TB, LO, true, true);                                \
+        else if (vec) RLG_GEMM_LAUNCH(TB, LO, true, false);                         \
+        else RLG_GEMM_LAUNCH(TB, LO, false, false);                                 \
+    } while (0)
     if (trans_b) {
         if (lower_only) RLG_GEMM_LAUNCH_V(true, true); else RLG_GEMM_LAUNCH_V(true, false);
     } else {
