@@ -342,6 +342,9 @@ def main():
         torch.cuda.empty_cache()
         result.update(extras(lib, torch, dev, sptr))
         result["batched"] = batched(lib, torch, dev)
+        # the serving loop at the configured rate and at a rate one GPU sustains
+        result["serving"] = {"offered_2000_per_s": serving_trace(lib, torch, dev, 2000, 1.0, 50),
+                             "offered_200_per_s": serving_trace(lib, torch, dev, 200, 2.0, 51)}
 
     # ------------------------------------------------------------ CPU baseline
     if not args.no_cpu:
@@ -495,6 +498,54 @@ def batched(lib, torch, dev, count=20000, nmin=8, nmax=512, seed=30):
                                      f"PartialPivLu / Cholesky decompose + solve per RHS, {dt:.1f}s"}
     log(f"CPU oracle, batched sample: {len(host)} systems in {dt:.1f}s, {len(host) / dt:.1f} solves/s")
     del A, B, A0, B0, P
+    torch.cuda.empty_cache()
+    return out
+
+
+def serving_trace(lib, torch, dev, rate, seconds, seed):
+    """Serving workload: Poisson arrivals at `rate` requests/s for `seconds`;
+    n log-uniform on [16, 4096]; 70 % LU (Gaussian) / 30 % Cholesky
+    (G G^T / n + I); one right-hand side; about 10 % of the requests keep
+    their factor resident and 20 % re-solve a new right-hand side against an
+    earlier kept factor; 64 GB admission budget.  Inputs are generated on the
+    device before the clock starts (as if uploaded on arrival).  Each trace
+    runs plain FP64 and mixed (FP32-factor + FP64 refinement)."""
+    from rulinalg_amd import serving
+    rng = np.random.default_rng(seed)
+    count = int(rate * seconds)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    specs, kept = [], []
+    for i in range(count):
+        if kept and rng.random() < 0.2:
+            src = specs[kept[int(rng.integers(len(kept)))]]
+            specs.append({"kind": src["kind"], "n": src["n"], "a": src["a"], "reuse": src["keep"]})
+            continue
+        n = int(np.exp(rng.uniform(np.log(16), np.log(4096))))
+        kind = serving.KIND_LU if rng.random() < 0.7 else serving.KIND_CHOL
+        m = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+        if kind == serving.KIND_CHOL:
+            m = m @ m.t().contiguous() / n + torch.eye(n, dtype=torch.float64, device=dev)
+        keep = i if rng.random() < 0.1 else None
+        if keep is not None:
+            kept.append(len(specs))
+        specs.append({"kind": kind, "n": n, "a": m.reshape(-1), "keep": keep, "reuse": None})
+    for sp in specs:
+        sp["b"] = torch.randn(sp["n"], dtype=torch.float64, device=dev, generator=g)
+    torch.cuda.synchronize()
+    out = {"rate_per_s": rate, "requests": count, "seconds_of_arrivals": seconds}
+    for mixed in (False, True):
+        reqs = [serving.Request(i, sp["kind"], sp["n"], 1, sp["a"], sp["b"], torch.empty_like(sp["b"]),
+                                float(arrivals[i]), keep_id=sp.get("keep"), reuse_id=sp.get("reuse"))
+                for i, sp in enumerate(specs)]
+        loop = serving.ServingLoop(lib, torch, dev, mem_budget_bytes=64 << 30, mixed=mixed)
+        loop.run(reqs)
+        loop.close()
+        s = serving.summarize(reqs)
+        out["mixed" if mixed else "plain"] = s
+        log(f"serving {rate}/s {'mixed' if mixed else 'plain'}: {s}")
+    del specs
     torch.cuda.empty_cache()
     return out
 
