@@ -163,11 +163,14 @@ __device__ __forceinline__ bool xwait(unsigned &spins, int *timeout) {
 //      to date with pivot j - 1 (the arithmetic the owners apply in place),
 //      then publishes the candidate (row index, |value|, the row) and, if it
 //      holds row j, row j;
-//   C. every thread applies pivot j - 1 to columns > j of its rows; barrier;
-//   D. wave 0 reads all G candidates and picks the same winner as every
-//      other workgroup, fetches the winning row unless it is its own and, if
-//      it holds the winning row but not row j, row j; swaps the rows it
-//      holds and puts the pivot row in LDS; barrier;
+//   C/D1. pivot j - 1 is applied to columns > j of every row; with other
+//      workgroups (G > 1, FP32) wave 0 meanwhile reads all G candidates, picks the
+//      same winner as every other workgroup and fetches the winning row
+//      unless it is its own (and, if it holds the winning row but not row j,
+//      row j), while waves 1-3 update all rows, wave 0's included; alone,
+//      every thread updates its own rows; barrier;
+//   D2. wave 0 swaps the rows it holds and puts the pivot row in LDS;
+//      barrier;
 //   E. every thread turns column j of its rows below j into multipliers and
 //      applies pivot j to column j + 1.
 // The pivot indices stay in LDS until the end: a global store before a
@@ -320,21 +323,17 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
             }
         }
 
-        // C. pivot j - 1 on the columns right of j
-        if (j > 0) {
-#pragma unroll
-            for (int h = 0; h < RPT; ++h) {
-                const int r = tid + h * PTHREADS;
-                if (r < rows && r0 + r >= j) elim_row_from(tile, pmprev, r, j, tile[r][j - 1]);
-            }
-        }
-        __syncthreads();
-
-        // D. the winner, the swap and the pivot row
-        if (wave == 0) {
-            int p = bi;
-            T pabs = bv;
-            if (G > 1 && live) {
+        // D1 / C.  With other workgroups to hear from, wave 0 waits for the
+        // exchange (D1) while waves 1-3 apply pivot j - 1 to every row,
+        // wave 0's included (C); alone, each wave updates its own rows.
+        // (FP32 only: with two rows per thread the hand-over spreads evenly;
+        // with one, a wave of 1-3 would do twice its work, measured slower;
+        // FP64 updates its rows first and then exchanges, after the barrier)
+        const bool w0_exchange = RPT > 1 && G > 1;  // uniform
+        int p = bi;
+        T pabs = bv;
+        auto exchange = [&]() {
+            if (live) {
                 const unsigned tag = (unsigned)(k + j + 1);
                 const xword *base = xch + (size_t)(j & 1) * XPAR * XS;
                 const xword *diag = base + (size_t)GMAX * XS;
@@ -400,6 +399,36 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
                 // after a timeout the result is discarded: keep indices in range
                 if (!live) { p = bi; pabs = bv; }
             }
+        };
+        if (wave == 0 && w0_exchange) exchange();
+        if (j > 0) {
+            if (!w0_exchange) {
+#pragma unroll
+                for (int h = 0; h < RPT; ++h) {
+                    const int r = tid + h * PTHREADS;
+                    if (r < rows && r0 + r >= j) elim_row_from(tile, pmprev, r, j, tile[r][j - 1]);
+                }
+            } else if (wave > 0) {
+                // own rows, then wave 0's 64 RPT rows spread over the 192 threads
+#pragma unroll
+                for (int h = 0; h < RPT; ++h) {
+                    const int r = tid + h * PTHREADS;
+                    if (r < rows && r0 + r >= j) elim_row_from(tile, pmprev, r, j, tile[r][j - 1]);
+                }
+                constexpr int XTRA = (64 * RPT + PTHREADS - 64 - 1) / (PTHREADS - 64);
+#pragma unroll
+                for (int q = 0; q < XTRA; ++q) {
+                    const int e = (tid - 64) + q * (PTHREADS - 64);
+                    const int r = (e % 64) + (e / 64) * PTHREADS;
+                    if (e < 64 * RPT && r < rows && r0 + r >= j) elim_row_from(tile, pmprev, r, j, tile[r][j - 1]);
+                }
+            }
+        }
+        __syncthreads();
+
+        // D2. (FP64: the exchange,) the swap and the pivot row
+        if (wave == 0) {
+            if (G > 1 && !w0_exchange) exchange();
             // swap rows j and p where this workgroup holds them (lanes w..63
             // carry zeros on both sides)
             if (p != j) {
