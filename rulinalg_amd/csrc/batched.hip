@@ -78,7 +78,9 @@ struct Smem {
 };
 
 constexpr int BNW = BTH / 64;  // waves per workgroup
-constexpr int TSTRIP = 4;      // 16 x 16 tiles per wave step of the trailing updates
+constexpr int TSTRIP = 4;      // 16 x 16 tiles per wave step of the Cholesky SYRK update
+constexpr int TSTRIP_LU = 8;   // and of the LU trailing update (measured 15 % faster than 4;
+                               // the SYRK's strips end at the diagonal, where 8 wastes tiles)
 
 // Copy the m x kb block at A[k.., k..] (ld n) into the LDS panel P (ld LD), or
 // back (TO_A); eight loads in flight per thread before their stores.  With
@@ -303,22 +305,22 @@ __device__ int lu_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
         }
         __syncthreads();
         // trailing update A22 -= L21 * U12 on MFMA tiles: a wave takes a strip
-        // of TSTRIP tiles along a tile row and issues all of the strip's
+        // of TSTRIP_LU tiles along a tile row and issues all of the strip's
         // global loads (C tiles and U12 fragments) before its MFMAs
         const int mt = m - kb;  // rows / cols of A22
         if (mt > 0) {
             using Acc = typename BMfma<T>::acc;
             constexpr int KS = NB / 4;
-            const int tiles = (mt + 15) / 16, strips = (tiles + TSTRIP - 1) / TSTRIP;
+            const int tiles = (mt + 15) / 16, strips = (tiles + TSTRIP_LU - 1) / TSTRIP_LU;
             const int lr = lane & 15, lk = lane >> 4;
             T *C = A + (long)(k + kb) * n + k + kb;
             const T *U = A + (long)k * n + k + kb;
             for (int t = wave; t < tiles * strips; t += BNW) {
-                const int ti = t / strips, tj0 = (t - ti * strips) * TSTRIP;
-                Acc acc[TSTRIP];
-                T bf[TSTRIP][KS];
+                const int ti = t / strips, tj0 = (t - ti * strips) * TSTRIP_LU;
+                Acc acc[TSTRIP_LU];
+                T bf[TSTRIP_LU][KS];
 #pragma unroll
-                for (int s = 0; s < TSTRIP; ++s) {
+                for (int s = 0; s < TSTRIP_LU; ++s) {
                     const int gj = (tj0 + s) * 16 + lr;
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
@@ -338,11 +340,11 @@ __device__ int lu_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
                     af[q] = (ai < mt && kk < kb) ? -P[(kb + ai) * LD + kk] : T(0);
                 }
 #pragma unroll
-                for (int s = 0; s < TSTRIP; ++s)
+                for (int s = 0; s < TSTRIP_LU; ++s)
 #pragma unroll
                     for (int q = 0; q < KS; ++q) acc[s] = BMfma<T>::op(af[q], bf[s][q], acc[s]);
 #pragma unroll
-                for (int s = 0; s < TSTRIP; ++s) {
+                for (int s = 0; s < TSTRIP_LU; ++s) {
                     const int gj = (tj0 + s) * 16 + lr;
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
