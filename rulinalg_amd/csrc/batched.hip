@@ -526,14 +526,26 @@ __device__ int chol_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
         panel_move<T, LD, true, true>(A, n, k, m, kb, P);
         // RHS: y1 = L11^-1 b1 (forward_substitution order per vector), then
         // b2 -= L21 y1
+        // (the block of each vector in registers: solved in place in memory,
+        // every read would wait for the previous step's store)
         for (int r = tid; r < nrhs; r += BTH) {
 #pragma clang fp contract(off)
             T *x = B + (long)r * n + k;
-            for (int i = 0; i < kb; ++i) {
-                T s = T(0);
-                for (int c = 0; c < i; ++c) s = s + P[i * LD + c] * x[c];
-                x[i] = (x[i] - s) / P[i * LD + i];
+            T xv[NB];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) xv[i] = i < kb ? x[i] : T(0);
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                if (i < kb) {
+                    T s = T(0);
+#pragma unroll
+                    for (int c = 0; c < i; ++c) s = s + P[i * LD + c] * xv[c];
+                    xv[i] = (xv[i] - s) / P[i * LD + i];
+                }
             }
+#pragma unroll
+            for (int i = 0; i < NB; ++i)
+                if (i < kb) x[i] = xv[i];
         }
         __syncthreads();
         const int mt = m - kb;
@@ -541,8 +553,13 @@ __device__ int chol_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
             for (int q = tid; q < mt * nrhs; q += BTH) {
                 const int r = q / mt, i = q - r * mt;
                 T *br = B + (long)r * n;
+                T yv[NB];
+#pragma unroll
+                for (int c = 0; c < NB; ++c) yv[c] = c < kb ? br[k + c] : T(0);
                 T s = br[k + kb + i];
-                for (int c = 0; c < kb; ++c) s = s - P[(kb + i) * LD + c] * br[k + c];
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+                    if (c < kb) s = s - P[(kb + i) * LD + c] * yv[c];
                 br[k + kb + i] = s;
             }
             // SYRK on lower tiles: A22 -= L21 L21^T, in strips of TSTRIP tiles
