@@ -734,12 +734,20 @@ __global__ __launch_bounds__(BTH) void k_batch_mixed(rlg_batch_item *__restrict_
             for (int q = 0; q < 8; ++q)
                 if (e0 + q * BTH < n * n) ms.a32[e0 + q * BTH] = (float)v[q];
         }
-        {
+        {   // four rows per wave step, so four rows' loads are in flight at once
             double rs = 0.0;
-            for (int i = wave; i < n; i += BNW) {
-                double s = 0.0;
-                for (int j = lane; j < n; j += 64) s += fabs(A[(long)i * n + j]);
-                rs = fmax(rs, wave_sum_f64(s));
+            for (int i0 = 4 * wave; i0 < n; i0 += 4 * BNW) {
+                double s[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int j = lane; j < n; j += 64) {
+                    double a[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) a[q] = i0 + q < n ? A[(long)(i0 + q) * n + j] : 0.0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) s[q] += fabs(a[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (i0 + q < n) rs = fmax(rs, wave_sum_f64(s[q]));
             }
             if (lane == 0) sm.red_v[wave] = rs;
         }
@@ -816,29 +824,43 @@ __global__ __launch_bounds__(BTH) void k_batch_mixed(rlg_batch_item *__restrict_
                 if (lane == 0) xmax[r] = xm;
             }
             __syncthreads();
-            // r = b - A x in FP64, one row per wave, four vectors per pass
-            // over the row; |r| per vector kept in the wave's own LDS slots
+            // r = b - A x in FP64: a wave takes four rows and four vectors per
+            // pass over the rows (lanes across the row), so four rows' loads
+            // are in flight at once; |r| per vector kept in the wave's own LDS
+            // slots.  Each (row, vector) sum keeps one fixed order.
             if (lane == 0) {
 #pragma unroll
                 for (int r = 0; r < BMAXR; ++r) rmax[wave * BMAXR + r] = 0.0;
             }
-            for (int i = wave; i < n; i += BNW) {
+            for (int i0 = 4 * wave; i0 < n; i0 += 4 * BNW) {
                 for (int rb = 0; rb < nrhs; rb += 4) {
-                    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-                    for (int j = lane; j < n; j += 64) {
-                        const double a = A[(long)i * n + j];
+                    double acc[4][4];
 #pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            if (rb + q < nrhs) acc[q] += a * X[(long)(rb + q) * n + j];
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) acc[u][q] = 0.0;
+                    for (int j = lane; j < n; j += 64) {
+                        double a[4], xv[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) a[u] = i0 + u < n ? A[(long)(i0 + u) * n + j] : 0.0;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) xv[q] = rb + q < nrhs ? X[(long)(rb + q) * n + j] : 0.0;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) acc[u][q] += a[u] * xv[q];
                     }
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        if (rb + q < nrhs) {
-                            const int r = rb + q;
-                            const double res = it.b[(long)r * n + i] - wave_sum_f64(acc[q]);
-                            if (lane == 0) {
-                                R[(long)r * n + i] = res;
-                                rmax[wave * BMAXR + r] = fmax(rmax[wave * BMAXR + r], fabs(res));
+                    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            if (i0 + u < n && rb + q < nrhs) {
+                                const int i = i0 + u, r = rb + q;
+                                const double res = it.b[(long)r * n + i] - wave_sum_f64(acc[u][q]);
+                                if (lane == 0) {
+                                    R[(long)r * n + i] = res;
+                                    rmax[wave * BMAXR + r] = fmax(rmax[wave * BMAXR + r], fabs(res));
+                                }
                             }
                         }
                     }
