@@ -1,0 +1,151 @@
+"""CPU: the serving loop's host logic (rulinalg_amd.serving) with a stub
+library: routing (batched launch for one-shot orders <= 512, per-system path
+otherwise), admission by the memory budget (RLG_E_OOM for a request that can
+never fit), least-recently-used eviction of resident factors, and reuse of a
+resident factor.  Device pointers are host pointers here; the stub records
+calls instead of computing."""
+import ctypes as C
+import types
+
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from rulinalg_amd import _lib, serving  # noqa: E402
+
+
+class StubLib:
+    def __init__(self):
+        self.calls = []
+        self.handles = {}
+        self.next = 1
+
+    def _h(self, out, kind, n):
+        h = self.next
+        self.next += 1
+        self.handles[h] = (kind, n)
+        out._obj.value = h
+        return 0
+
+    # memory
+    def rlg_dev_upload(self, dst, src, n):
+        C.memmove(dst, src, n)
+        return 0
+
+    def rlg_dev_download(self, dst, src, n):
+        C.memmove(dst, src, n)
+        return 0
+
+    def rlg_synchronize(self):
+        return 0
+
+    # solvers
+    def rlg_batch_solve_f64_dev(self, count, items, stream):
+        arr = (_lib.BatchItem * count).from_address(items)
+        self.calls.append(("batch", [arr[i].n for i in range(count)]))
+        for i in range(count):
+            arr[i].status = 0
+        return 0
+
+    def rlg_batch_solve_mixed_dev(self, count, items, max_iter, tol, fb, stream):
+        return self.rlg_batch_solve_f64_dev(count, items, stream)
+
+    def rlg_lu_decompose_f64_dev(self, n, a, lda, stream, out):
+        self.calls.append(("lu", n))
+        return self._h(out, "lu", n)
+
+    def rlg_chol_decompose_f64_dev(self, n, a, lda, stream, out):
+        self.calls.append(("chol", n))
+        return self._h(out, "chol", n)
+
+    def rlg_lu_solve_f64_dev(self, h, nrhs, b, ldb, x, ldx, stream):
+        self.calls.append(("lu_solve", self.handles[h.value if hasattr(h, "value") else h][1]))
+        return 0
+
+    rlg_chol_solve_f64_dev = rlg_lu_solve_f64_dev
+
+    def rlg_lu_free(self, h):
+        self.calls.append(("free", h.value if hasattr(h, "value") else h))
+
+    rlg_chol_free = rlg_lu_free
+
+    def rlg_backward_error_f64_dev(self, n, a, lda, nrhs, b, ldb, x, ldx, eta, stream):
+        eta._obj.value = 1e-16
+        return 0
+
+    def rlg_solve_mixed_f64_dev(self, n, a, lda, nrhs, b, ldb, x, ldx, opts, rep, stream):
+        self.calls.append(("mixed", n))
+        rep._obj.backward_error = 1e-15
+        return 0
+
+
+def torch_cpu():
+    """torch with the device calls the loop makes mapped onto the CPU."""
+    t = types.SimpleNamespace()
+    t.cat = torch.cat
+    t.empty = lambda *a, **k: torch.empty(*a, **{kk: v for kk, v in k.items() if kk != "device"})
+    t.uint8, t.int64, t.float64 = torch.uint8, torch.int64, torch.float64
+    t.cuda = types.SimpleNamespace(synchronize=lambda: None)
+    return t
+
+
+def req(rid, kind, n, arrival=0.0, **kw):
+    a = torch.zeros(n * n, dtype=torch.float64)
+    b = torch.ones(n, dtype=torch.float64)
+    return serving.Request(rid, kind, n, 1, a, b, torch.empty_like(b), arrival, **kw)
+
+
+def test_routing_batch_and_single():
+    lib = StubLib()
+    loop = serving.ServingLoop(lib, torch_cpu(), None)
+    reqs = [req(0, serving.KIND_LU, 100), req(1, serving.KIND_CHOL, 512), req(2, serving.KIND_LU, 513),
+            req(3, serving.KIND_CHOL, 900)]
+    loop.run(reqs)
+    assert [r.path for r in reqs] == ["batched", "batched", "single", "single"]
+    assert all(r.status == 0 and r.eta is not None for r in reqs)
+    assert ("batch", [100, 512]) in lib.calls
+    assert ("lu", 513) in lib.calls and ("chol", 900) in lib.calls
+    # one-shot per-system handles are freed
+    assert sum(1 for c in lib.calls if c[0] == "free") == 2
+    s = serving.summarize(reqs)
+    assert s["completed"] == 4 and s["paths"] == {"batched": 2, "single": 2}
+
+
+def test_budget_rejects_what_never_fits():
+    lib = StubLib()
+    loop = serving.ServingLoop(lib, torch_cpu(), None, mem_budget_bytes=1 << 20)
+    big, small = req(0, serving.KIND_LU, 400), req(1, serving.KIND_LU, 100, arrival=0.0)
+    loop.run([big, small])
+    assert big.status == serving.RLG_E_OOM and big.path == "rejected"
+    assert small.status == 0
+
+
+def test_resident_reuse_and_lru_eviction():
+    lib = StubLib()
+    # room for two resident 128-order factors (16 n^2 bytes each) plus a pass
+    budget = 2 * 16 * 128 * 128 + 8 * 128 * 129
+    loop = serving.ServingLoop(lib, torch_cpu(), None, mem_budget_bytes=budget)
+    k1 = req(0, serving.KIND_LU, 128, keep_id=1)
+    k2 = req(1, serving.KIND_LU, 128, arrival=0.001, keep_id=2)
+    loop.run([k1, k2])
+    assert set(loop.resident) == {1, 2}
+    use1 = req(2, serving.KIND_LU, 128, reuse_id=1)
+    loop.run([use1])  # factor 1 becomes the most recently used
+    assert use1.path == "reuse" and use1.status == 0
+    k3 = req(3, serving.KIND_LU, 128, keep_id=3)
+    loop.run([k3])  # needs room: factor 2, the least recently used, goes
+    assert set(loop.resident) == {1, 3}
+    late = req(4, serving.KIND_LU, 128, reuse_id=2)
+    loop.run([late])
+    assert late.status == 1  # InvalidArg: no longer resident
+    loop.close()
+    assert not loop.resident and loop.resident_bytes == 0
+
+
+def test_mixed_mode_routes_large_lu_to_the_mixed_solve():
+    lib = StubLib()
+    loop = serving.ServingLoop(lib, torch_cpu(), None, mixed=True)
+    reqs = [req(0, serving.KIND_LU, 600), req(1, serving.KIND_CHOL, 600), req(2, serving.KIND_LU, 64)]
+    loop.run(reqs)
+    assert [r.path for r in reqs] == ["single_mixed", "single", "batched_mixed"]
+    assert ("mixed", 600) in lib.calls
