@@ -60,11 +60,19 @@ __global__ __launch_bounds__(64) void k_potrf_diag(T *__restrict__ A, long lda, 
                                                    int64_t *__restrict__ info, int n) {
 #pragma clang fp contract(off)
     __shared__ T a[CNB][CNB + 1];
+    __shared__ T sdiv[CNB];
     const int t = threadIdx.x;
-    for (int r = 0; r < jb; ++r)
-        if (t <= r) a[r][t] = A[(long)(j0 + r) * lda + j0 + t];
+    {   // all of the block's loads in flight before their LDS stores
+        T v[CNB];
+#pragma unroll
+        for (int r = 0; r < CNB; ++r) v[r] = (r < jb && t <= r) ? A[(long)(j0 + r) * lda + j0 + t] : T(0);
+#pragma unroll
+        for (int r = 0; r < CNB; ++r)
+            if (r < jb && t <= r) a[r][t] = v[r];
+    }
     __syncthreads();
     bool failed = false;
+    int jend = jb;  // columns whose divisor was computed
     for (int j = 0; j < jb; ++j) {
         // a[k][j] -= dot(a[k][0..j], a[j][0..j]) for k >= j (cholesky.rs:141-148)
         if (t >= j && t < jb && j > 0) {
@@ -79,10 +87,12 @@ __global__ __launch_bounds__(64) void k_potrf_diag(T *__restrict__ A, long lda, 
         }
         const T div = sqrt(diag);
         if (t >= j && t < jb) a[t][j] = a[t][j] / div;
-        if (t == 0) divisor[j0 + j] = div;
+        // kept in LDS: a global store here would make the barrier wait for it
+        if (t == 0) sdiv[j] = div;
         __syncthreads();
-        if (failed) break;  // the reference stops at the first failure
+        if (failed) { jend = j + 1; break; }  // the reference stops at the first failure
     }
+    if (t < jend) divisor[j0 + t] = sdiv[t];
     for (int r = 0; r < jb; ++r)
         if (t <= r) A[(long)(j0 + r) * lda + j0 + t] = a[r][t];
 }
@@ -98,10 +108,21 @@ __global__ __launch_bounds__(64) void k_chol_trsm(T *__restrict__ A, long lda, i
     __shared__ T X[64][CNB + 1];
     const int t = threadIdx.x;
     const int row = r0 + blockIdx.x * 64 + t;
-    for (int r = 0; r < jb; ++r)
-        if (t <= r) L[r][t] = A[(long)(j0 + r) * lda + j0 + t];
     const bool act = row < n;
-    for (int c = 0; c < jb; ++c) X[t][c] = act ? A[(long)row * lda + j0 + c] : T(0);
+    {   // all loads in flight before their LDS stores (L11, then this row)
+        T v[CNB];
+#pragma unroll
+        for (int r = 0; r < CNB; ++r) v[r] = (r < jb && t <= r) ? A[(long)(j0 + r) * lda + j0 + t] : T(0);
+        T x[CNB];
+#pragma unroll
+        for (int c = 0; c < CNB; ++c) x[c] = (act && c < jb) ? A[(long)row * lda + j0 + c] : T(0);
+#pragma unroll
+        for (int r = 0; r < CNB; ++r)
+            if (r < jb && t <= r) L[r][t] = v[r];
+#pragma unroll
+        for (int c = 0; c < CNB; ++c)
+            if (c < jb) X[t][c] = x[c];
+    }
     __syncthreads();
     if (act) {
         for (int c = 0; c < jb; ++c) {
