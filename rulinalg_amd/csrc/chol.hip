@@ -26,29 +26,47 @@ template <class T> __device__ __forceinline__ T ceps();
 template <> __device__ __forceinline__ double ceps<double>() { return 2.220446049250313e-16; }
 template <> __device__ __forceinline__ float ceps<float>() { return 1.1920929e-7f; }
 
-// utils::dot (src/utils.rs:20-51) of u[0..len) and v[0..len) where u has
-// unit stride and v has stride vs.
+// utils::dot (src/utils.rs:20-51) of u[0..len) and v[0..len), both in LDS
+// with unit stride and at least 64 readable entries, len <= 63: partial sums
+// p0..p7 over chunks of eight, combined as s + p0 + p4 + p1 + p5 + p2 + p6 +
+// p3 + p7, then the tail in order.  The next chunk (and finally the tail
+// chunk, read whole) is loaded before the current one is accumulated, so
+// each chunk costs one LDS round trip overlapped with the previous one's
+// arithmetic; entries past len are read but never accumulated.
 template <class T>
-__device__ __forceinline__ T dot8(const T *u, const T *v, int vs, int len) {
+__device__ __forceinline__ T dot8(const T *u, const T *v, int len) {
 #pragma clang fp contract(off)
-    T p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0, p6 = 0, p7 = 0;
-    int off = 0;
-    for (; len - off >= 8; off += 8) {
-        p0 = p0 + u[off + 0] * v[(off + 0) * vs];
-        p1 = p1 + u[off + 1] * v[(off + 1) * vs];
-        p2 = p2 + u[off + 2] * v[(off + 2) * vs];
-        p3 = p3 + u[off + 3] * v[(off + 3) * vs];
-        p4 = p4 + u[off + 4] * v[(off + 4) * vs];
-        p5 = p5 + u[off + 5] * v[(off + 5) * vs];
-        p6 = p6 + u[off + 6] * v[(off + 6) * vs];
-        p7 = p7 + u[off + 7] * v[(off + 7) * vs];
+    T p[8], cu[8], cv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        p[q] = T(0);
+        cu[q] = u[q];
+        cv[q] = v[q];
+    }
+    const int m8 = len & ~7;
+    for (int off = 0; off < m8; off += 8) {
+        T nu[8], nv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {  // off + 8 <= m8 <= 56: in bounds
+            nu[q] = u[off + 8 + q];
+            nv[q] = v[off + 8 + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) p[q] = p[q] + cu[q] * cv[q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            cu[q] = nu[q];
+            cv[q] = nv[q];
+        }
     }
     T s = T(0);
-    s = s + p0 + p4;
-    s = s + p1 + p5;
-    s = s + p2 + p6;
-    s = s + p3 + p7;
-    for (; off < len; ++off) s = s + u[off] * v[off * vs];
+    s = s + p[0] + p[4];
+    s = s + p[1] + p[5];
+    s = s + p[2] + p[6];
+    s = s + p[3] + p[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+        if (q < len - m8) s = s + cu[q] * cv[q];
     return s;
 }
 
@@ -76,7 +94,7 @@ __global__ __launch_bounds__(64) void k_potrf_diag(T *__restrict__ A, long lda, 
     for (int j = 0; j < jb; ++j) {
         // a[k][j] -= dot(a[k][0..j], a[j][0..j]) for k >= j (cholesky.rs:141-148)
         if (t >= j && t < jb && j > 0) {
-            const T d = dot8(&a[t][0], &a[j][0], 1, j);
+            const T d = dot8(&a[t][0], &a[j][0], j);
             a[t][j] = a[t][j] - d;
         }
         __syncthreads();
@@ -99,37 +117,45 @@ __global__ __launch_bounds__(64) void k_potrf_diag(T *__restrict__ A, long lda, 
 
 // Rows i in [r0, n) of the block column j0..j0+jb: solve x L11^T = b with
 // the same recurrence as the diagonal block: x[c] = (b[c] - dot(x[0..c],
-// L11[c][0..c])) / divisor[c].  One thread per row, 64 rows per workgroup.
+// L11[c][0..c])) / divisor[c].  One thread per row, 64 rows per workgroup;
+// the rows are read and written whole (lane t holds column t), so every
+// access to A is coalesced, and transposed through LDS.
 template <class T>
 __global__ __launch_bounds__(64) void k_chol_trsm(T *__restrict__ A, long lda, int j0, int jb, int r0,
                                                   int n, const T *__restrict__ divisor) {
 #pragma clang fp contract(off)
     __shared__ T L[CNB][CNB + 1];
     __shared__ T X[64][CNB + 1];
+    __shared__ T sdiv[CNB];
     const int t = threadIdx.x;
-    const int row = r0 + blockIdx.x * 64 + t;
-    const bool act = row < n;
-    {   // all loads in flight before their LDS stores (L11, then this row)
+    const int rb = r0 + blockIdx.x * 64;  // first row of this workgroup
+    const int nr = n - rb < 64 ? n - rb : 64;
+    {   // all loads in flight before their LDS stores (L11, divisors, rows)
         T v[CNB];
 #pragma unroll
         for (int r = 0; r < CNB; ++r) v[r] = (r < jb && t <= r) ? A[(long)(j0 + r) * lda + j0 + t] : T(0);
-        T x[CNB];
+        T x[64];
 #pragma unroll
-        for (int c = 0; c < CNB; ++c) x[c] = (act && c < jb) ? A[(long)row * lda + j0 + c] : T(0);
+        for (int r = 0; r < 64; ++r) x[r] = (r < nr && t < jb) ? A[(long)(rb + r) * lda + j0 + t] : T(0);
+        const T dv = t < jb ? divisor[j0 + t] : T(1);
 #pragma unroll
         for (int r = 0; r < CNB; ++r)
             if (r < jb && t <= r) L[r][t] = v[r];
 #pragma unroll
-        for (int c = 0; c < CNB; ++c)
-            if (c < jb) X[t][c] = x[c];
+        for (int r = 0; r < 64; ++r) X[r][t] = x[r];
+        sdiv[t] = dv;
     }
     __syncthreads();
-    if (act) {
+    if (t < nr) {
         for (int c = 0; c < jb; ++c) {
-            const T d = c ? dot8(&X[t][0], &L[c][0], 1, c) : T(0);
-            X[t][c] = (X[t][c] - d) / divisor[j0 + c];
+            const T d = c ? dot8(&X[t][0], &L[c][0], c) : T(0);
+            X[t][c] = (X[t][c] - d) / sdiv[c];
         }
-        for (int c = 0; c < jb; ++c) A[(long)row * lda + j0 + c] = X[t][c];
+    }
+    __syncthreads();
+    if (t < jb) {
+#pragma unroll 8
+        for (int r = 0; r < nr; ++r) A[(long)(rb + r) * lda + j0 + t] = X[r][t];
     }
 }
 
