@@ -15,6 +15,8 @@
 // reference.  Only entries on or below the diagonal are read; entries above
 // it may be overwritten with intermediate values (the reference's unpack
 // zeroes them, cholesky.rs:237-246).
+#include <utility>
+
 #include "kernels.hpp"
 
 namespace rlg {
@@ -26,38 +28,33 @@ template <class T> __device__ __forceinline__ T ceps();
 template <> __device__ __forceinline__ double ceps<double>() { return 2.220446049250313e-16; }
 template <> __device__ __forceinline__ float ceps<float>() { return 1.1920929e-7f; }
 
-// utils::dot (src/utils.rs:20-51) of u[0..len) and v[0..len), both in LDS
-// with unit stride and at least 64 readable entries, len <= 63: partial sums
-// p0..p7 over chunks of eight, combined as s + p0 + p4 + p1 + p5 + p2 + p6 +
-// p3 + p7, then the tail in order.  The next chunk (and finally the tail
-// chunk, read whole) is loaded before the current one is accumulated, so
-// each chunk costs one LDS round trip overlapped with the previous one's
-// arithmetic; entries past len are read but never accumulated.
-template <class T>
-__device__ __forceinline__ T dot8(const T *u, const T *v, int len) {
+// Calls f(std::integral_constant<int, I>) for I = 0 .. N-1 in order, so a
+// loop body sees its index as a compile-time constant and registers indexed
+// by it stay registers.
+template <class F, int... Is>
+__device__ __forceinline__ void static_for_impl(F &f, std::integer_sequence<int, Is...>) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// utils::dot (src/utils.rs:20-51) of x[0..LEN) held in registers and
+// l[0..LEN) in LDS, LEN known at compile time: partial sums p0..p7 over
+// chunks of eight, combined as s + p0 + p4 + p1 + p5 + p2 + p6 + p3 + p7,
+// then the tail in order.
+template <int LEN, class T>
+__device__ __forceinline__ T dot8_reg(const T (&x)[CNB], const T *l) {
 #pragma clang fp contract(off)
-    T p[8], cu[8], cv[8];
+    constexpr int M8 = LEN / 8 * 8;
+    T p[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        p[q] = T(0);
-        cu[q] = u[q];
-        cv[q] = v[q];
-    }
-    const int m8 = len & ~7;
-    for (int off = 0; off < m8; off += 8) {
-        T nu[8], nv[8];
+    for (int q = 0; q < 8; ++q) p[q] = T(0);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {  // off + 8 <= m8 <= 56: in bounds
-            nu[q] = u[off + 8 + q];
-            nv[q] = v[off + 8 + q];
-        }
+    for (int off = 0; off < M8; off += 8) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) p[q] = p[q] + cu[q] * cv[q];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            cu[q] = nu[q];
-            cv[q] = nv[q];
-        }
+        for (int q = 0; q < 8; ++q) p[q] = p[q] + x[off + q] * l[off + q];
     }
     T s = T(0);
     s = s + p[0] + p[4];
@@ -65,8 +62,7 @@ __device__ __forceinline__ T dot8(const T *u, const T *v, int len) {
     s = s + p[2] + p[6];
     s = s + p[3] + p[7];
 #pragma unroll
-    for (int q = 0; q < 7; ++q)
-        if (q < len - m8) s = s + cu[q] * cv[q];
+    for (int o = M8; o < LEN; ++o) s = s + x[o] * l[o];
     return s;
 }
 
@@ -89,27 +85,36 @@ __global__ __launch_bounds__(64) void k_potrf_diag(T *__restrict__ A, long lda, 
             if (r < jb && t <= r) a[r][t] = v[r];
     }
     __syncthreads();
+    T r[CNB];  // row t, finished entries replace the original ones
+#pragma unroll
+    for (int c = 0; c < CNB; ++c) r[c] = c <= t ? a[t][c] : T(0);
     bool failed = false;
     int jend = jb;  // columns whose divisor was computed
-    for (int j = 0; j < jb; ++j) {
-        // a[k][j] -= dot(a[k][0..j], a[j][0..j]) for k >= j (cholesky.rs:141-148)
-        if (t >= j && t < jb && j > 0) {
-            const T d = dot8(&a[t][0], &a[j][0], j);
-            a[t][j] = a[t][j] - d;
+    static_for<CNB>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        if (j < jb && !failed) {  // the reference stops at the first failure
+            // a[k][j] -= dot(a[k][0..j], a[j][0..j]) for k >= j (cholesky.rs:141-148)
+            if (j > 0 && t >= j && t < jb) {
+                r[j] = r[j] - dot8_reg<j>(r, &a[j][0]);
+                a[t][j] = r[j];
+            }
+            __syncthreads();
+            const T diag = a[j][j];
+            if (fabs(diag) < ceps<T>() || diag < T(0)) {
+                if (t == 0 && *info == (int64_t)n) *info = j0 + j;
+                failed = true;
+                jend = j + 1;
+            }
+            const T div = sqrt(diag);
+            if (t >= j && t < jb) {
+                r[j] = r[j] / div;
+                a[t][j] = r[j];
+            }
+            // kept in LDS: a global store here would make the barrier wait for it
+            if (t == 0) sdiv[j] = div;
+            __syncthreads();
         }
-        __syncthreads();
-        const T diag = a[j][j];
-        if (fabs(diag) < ceps<T>() || diag < T(0)) {
-            if (t == 0 && *info == (int64_t)n) *info = j0 + j;
-            failed = true;
-        }
-        const T div = sqrt(diag);
-        if (t >= j && t < jb) a[t][j] = a[t][j] / div;
-        // kept in LDS: a global store here would make the barrier wait for it
-        if (t == 0) sdiv[j] = div;
-        __syncthreads();
-        if (failed) { jend = j + 1; break; }  // the reference stops at the first failure
-    }
+    });
     if (t < jend) divisor[j0 + t] = sdiv[t];
     for (int r = 0; r < jb; ++r)
         if (t <= r) A[(long)(j0 + r) * lda + j0 + t] = a[r][t];
@@ -146,12 +151,15 @@ __global__ __launch_bounds__(64) void k_chol_trsm(T *__restrict__ A, long lda, i
         sdiv[t] = dv;
     }
     __syncthreads();
-    if (t < nr) {
-        for (int c = 0; c < jb; ++c) {
-            const T d = c ? dot8(&X[t][0], &L[c][0], c) : T(0);
-            X[t][c] = (X[t][c] - d) / sdiv[c];
-        }
-    }
+    T x[CNB];  // this thread's row
+#pragma unroll
+    for (int c = 0; c < CNB; ++c) x[c] = X[t][c];
+    static_for<CNB>([&](auto cc) __attribute__((always_inline)) {
+        constexpr int c = decltype(cc)::value;
+        if (c < jb) x[c] = (x[c] - (c ? dot8_reg<c>(x, &L[c][0]) : T(0))) / sdiv[c];
+    });
+#pragma unroll
+    for (int c = 0; c < CNB; ++c) X[t][c] = x[c];
     __syncthreads();
     if (t < jb) {
 #pragma unroll 8
