@@ -1,8 +1,9 @@
 // Blocked Cholesky (Cholesky::decompose, src/matrix/decomposition/cholesky.rs:
 // 116-169), lower triangle only, and its solve (cholesky.rs:194-202).
 //
-// Outer blocks of CONB = 256 columns end in one SYRK-shaped MFMA GEMM of
-// depth 256 on the lower tiles of the trailing matrix.  Inside an outer
+// Outer blocks of CONB = 512 columns end in one SYRK-shaped MFMA GEMM of
+// depth 512 on the lower tiles of the trailing matrix (512 measured faster
+// than 128 or 256 at order 16384, and wider blocks no faster).  Inside an outer
 // block, 64-column blocks are processed left to right:
 //   1. the diagonal block is factored by one workgroup in LDS with the
 //      reference's left-looking "gaxpy" recurrence: a[k][j] -= dot(row k,
@@ -22,7 +23,7 @@
 namespace rlg {
 
 constexpr int CNB = 64;
-constexpr int CONB = 256;
+constexpr int CONB = 512;
 
 template <class T> __device__ __forceinline__ T ceps();
 template <> __device__ __forceinline__ double ceps<double>() { return 2.220446049250313e-16; }
