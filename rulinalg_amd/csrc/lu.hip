@@ -34,6 +34,7 @@ constexpr int PR = 256;
 template <class T> struct PanelRows { static constexpr int value = PR; };
 template <> struct PanelRows<float> { static constexpr int value = 2 * PR; };  // same 133 KB
 constexpr int ONB = 256;   // outer block: depth of the trailing GEMM
+constexpr int SWB = 256;   // swaps per k_laswp launch (its workgroup size)
 constexpr int PTHREADS = 256;
 // At most this many workgroups per panel: one per CU of the device and never
 // more than the exchange area holds.
@@ -519,7 +520,7 @@ __device__ __forceinline__ bool swap_last(const int *P, int x, int t, int cnt) {
 // The list into LDS (dst[], src[]) by the 256 threads of a workgroup, in
 // the order above; returns its length (at most 2 cnt).  P is scratch.
 __device__ int swap_map_lds(const int64_t *__restrict__ ipiv, int k0, int cnt, int *P, int *dst,
-                            int *src, int (*wcount)[ONB / 64]) {
+                            int *src, int (*wcount)[SWB / 64]) {
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     if (t < cnt) P[t] = (int)ipiv[k0 + t];
     __syncthreads();
@@ -539,7 +540,7 @@ __device__ int swap_map_lds(const int64_t *__restrict__ ipiv, int k0, int cnt, i
     __syncthreads();
     int off_in = 0, tot_in = 0, off_out = 0, tot_out = 0;
 #pragma unroll
-    for (int q = 0; q < ONB / 64; ++q) {
+    for (int q = 0; q < SWB / 64; ++q) {
         if (q < wave) { off_in += wcount[0][q]; off_out += wcount[1][q]; }
         tot_in += wcount[0][q];
         tot_out += wcount[1][q];
@@ -561,19 +562,19 @@ __device__ int swap_map_lds(const int64_t *__restrict__ ipiv, int k0, int cnt, i
     return tot_in + tot_out;
 }
 
-// Applies the swaps ipiv[k0 .. k0+cnt) (cnt <= ONB) to columns [c0, c1) and
+// Applies the swaps ipiv[k0 .. k0+cnt) (cnt <= SWB) to columns [c0, c1) and
 // [c2, c3).  Every workgroup derives the row map itself (a few microseconds
 // of its 256 threads, where a separate launch would cost more), then owns
 // LCOLS columns: it reads every source row of the map into LDS, then writes
 // the destinations, so the in-place permutation needs no ordering.
 constexpr int LCOLS = 32;
 template <class T>
-__global__ __launch_bounds__(ONB) void k_laswp(T *__restrict__ A, long lda, const int64_t *__restrict__ ipiv,
+__global__ __launch_bounds__(SWB) void k_laswp(T *__restrict__ A, long lda, const int64_t *__restrict__ ipiv,
                                                int k0, int cnt_sw, int c0, int c1, int c2, int c3) {
-    __shared__ T stage[2 * ONB][LCOLS];
-    __shared__ int dst[2 * ONB], src[2 * ONB];
-    __shared__ int P[ONB];
-    __shared__ int wcount[2][ONB / 64];
+    __shared__ T stage[2 * SWB][LCOLS];
+    __shared__ int dst[2 * SWB], src[2 * SWB];
+    __shared__ int P[SWB];
+    __shared__ int wcount[2][SWB / 64];
     const int cnt = swap_map_lds(ipiv, k0, cnt_sw, P, dst, src, wcount);
     if (cnt == 0) return;
     // column block -> absolute columns
@@ -584,7 +585,7 @@ __global__ __launch_bounds__(ONB) void k_laswp(T *__restrict__ A, long lda, cons
     const int col = lc < w1 ? c0 + lc : c2 + (lc - w1);
     const bool okc = lc < w1 + (c3 - c2);
     // eight loads in flight per thread before their LDS stores
-    constexpr int RG = ONB / LCOLS;
+    constexpr int RG = SWB / LCOLS;
     for (int i0 = rg; i0 < cnt; i0 += 8 * RG) {
         T v[8];
 #pragma unroll
@@ -794,15 +795,16 @@ static int laswp(hipStream_t s, T *a, size_t lda, const int64_t *ipiv, size_t k0
                  size_t c0, size_t c1, size_t c2, size_t c3) {
     const size_t ncols = (c1 - c0) + (c3 - c2);
     if (ncols == 0 || cnt == 0) return RLG_OK;
-    if (cnt > (size_t)ONB) {
-        set_error("row swap block of %zu rows exceeds %d", cnt, ONB);
-        return RLG_E_INVALID_ARG;
+    // a longer sequence is applied SWB swaps at a time, in order: the same
+    // sequence of transpositions
+    for (size_t off = 0; off < cnt; off += SWB) {
+        const size_t c = cnt - off < (size_t)SWB ? cnt - off : SWB;
+        // algorithmic bytes: at most 2 c rows change, each read and written
+        RLG_PROF_W("laswp", s, 2.0 * 2.0 * (double)c * (double)ncols * sizeof(T));
+        hipLaunchKernelGGL(k_laswp<T>, dim3(ceil_div(ncols, LCOLS)), dim3(SWB), 0, s, a, (long)lda,
+                           ipiv, (int)(k0 + off), (int)c, (int)c0, (int)c1, (int)c2, (int)c3);
+        RLG_LAUNCHED();
     }
-    // algorithmic bytes: at most 2 cnt rows change, each read and written
-    RLG_PROF_W("laswp", s, 2.0 * 2.0 * (double)cnt * (double)ncols * sizeof(T));
-    hipLaunchKernelGGL(k_laswp<T>, dim3(ceil_div(ncols, LCOLS)), dim3(ONB), 0, s, a, (long)lda, ipiv,
-                       (int)k0, (int)cnt, (int)c0, (int)c1, (int)c2, (int)c3);
-    RLG_LAUNCHED();
     return RLG_OK;
 }
 
