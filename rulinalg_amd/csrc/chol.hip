@@ -5,13 +5,14 @@
 // depth 512 on the lower tiles of the trailing matrix (512 measured faster
 // than 128 or 256 at order 16384, and wider blocks no faster).  Inside an outer
 // block, 64-column blocks are processed left to right:
-//   1. the diagonal block is factored by one workgroup in LDS with the
+//   1. one launch (k_chol_block) factors the diagonal block with the
 //      reference's left-looking "gaxpy" recurrence: a[k][j] -= dot(row k,
 //      row j) over the block's previous columns in utils::dot's eight-lane
 //      order, the |d| < eps and d < 0 checks, and division by sqrt(d);
-//   2. the rows below are solved against it with the same recurrence
-//      (a triangular solve X L^T = B, row by row);
-//   3. a depth-64 GEMM updates the remaining columns of the outer block.
+//      in the same launch the rows below are solved against it with the
+//      same recurrence (a triangular solve X L^T = B, row by row), one
+//      column behind;
+//   2. a depth-64 GEMM updates the remaining columns of the outer block.
 // A matrix of order <= 64 is therefore factored bit-identically to the
 // reference.  Only entries on or below the diagonal are read; entries above
 // it may be overwritten with intermediate values (the reference's unpack
@@ -67,102 +68,99 @@ __device__ __forceinline__ T dot8_reg(const T (&x)[CNB], const T *l) {
     return s;
 }
 
-// Factor the jb x jb diagonal block at (j0, j0).  divisor[j0 + j] receives
-// sqrt(d_j), the value the reference divides the column by.
+// One 64-column block column j0 .. j0+jb: the diagonal block and the rows
+// below it, in one launch of two-wave workgroups.
+//
+// Wave 0 factors the jb x jb diagonal block with the reference's
+// left-looking recurrence (cholesky.rs:135-163): a[k][j] -= dot(a[k][0..j],
+// a[j][0..j]) for k >= j in utils::dot order, the |d| < eps and d < 0
+// checks, then division by sqrt(d).  Every workgroup does this redundantly
+// (the same arithmetic, so the same values); workgroup 0 writes the block,
+// the divisors and the failure index.  Wave 1 solves the workgroup's 64
+// rows below the block, x L11^T = b, with the same recurrence,
+// x[c] = (b[c] - dot(x[0..c], L11[c][0..c])) / divisor[c], one column
+// behind wave 0: column c needs only row c of L11 and divisor c, both final
+// once wave 0 has finished column c.  Each thread keeps its row in
+// registers (the column loop is expanded at compile time, so every index is
+// a constant) and reads the broadcast row of L11 from LDS.  The rows below
+// are read and written whole (coalesced) and transposed through LDS.
 template <class T>
-__global__ __launch_bounds__(64) void k_potrf_diag(T *__restrict__ A, long lda, int j0, int jb,
-                                                   T *__restrict__ divisor,
-                                                   int64_t *__restrict__ info, int n) {
+__global__ __launch_bounds__(128) void k_chol_block(T *__restrict__ A, long lda, int j0, int jb,
+                                                    int r0, int n, T *__restrict__ divisor,
+                                                    int64_t *__restrict__ info) {
 #pragma clang fp contract(off)
     __shared__ T a[CNB][CNB + 1];
-    __shared__ T sdiv[CNB];
-    const int t = threadIdx.x;
-    {   // all of the block's loads in flight before their LDS stores
-        T v[CNB];
-#pragma unroll
-        for (int r = 0; r < CNB; ++r) v[r] = (r < jb && t <= r) ? A[(long)(j0 + r) * lda + j0 + t] : T(0);
-#pragma unroll
-        for (int r = 0; r < CNB; ++r)
-            if (r < jb && t <= r) a[r][t] = v[r];
-    }
-    __syncthreads();
-    T r[CNB];  // row t, finished entries replace the original ones
-#pragma unroll
-    for (int c = 0; c < CNB; ++c) r[c] = c <= t ? a[t][c] : T(0);
-    bool failed = false;
-    int jend = jb;  // columns whose divisor was computed
-    static_for<CNB>([&](auto jc) __attribute__((always_inline)) {
-        constexpr int j = decltype(jc)::value;
-        if (j < jb && !failed) {  // the reference stops at the first failure
-            // a[k][j] -= dot(a[k][0..j], a[j][0..j]) for k >= j (cholesky.rs:141-148)
-            if (j > 0 && t >= j && t < jb) {
-                r[j] = r[j] - dot8_reg<j>(r, &a[j][0]);
-                a[t][j] = r[j];
-            }
-            __syncthreads();
-            const T diag = a[j][j];
-            if (fabs(diag) < ceps<T>() || diag < T(0)) {
-                if (t == 0 && *info == (int64_t)n) *info = j0 + j;
-                failed = true;
-                jend = j + 1;
-            }
-            const T div = sqrt(diag);
-            if (t >= j && t < jb) {
-                r[j] = r[j] / div;
-                a[t][j] = r[j];
-            }
-            // kept in LDS: a global store here would make the barrier wait for it
-            if (t == 0) sdiv[j] = div;
-            __syncthreads();
-        }
-    });
-    if (t < jend) divisor[j0 + t] = sdiv[t];
-    for (int r = 0; r < jb; ++r)
-        if (t <= r) A[(long)(j0 + r) * lda + j0 + t] = a[r][t];
-}
-
-// Rows i in [r0, n) of the block column j0..j0+jb: solve x L11^T = b with
-// the same recurrence as the diagonal block: x[c] = (b[c] - dot(x[0..c],
-// L11[c][0..c])) / divisor[c].  One thread per row, 64 rows per workgroup;
-// the rows are read and written whole (lane t holds column t), so every
-// access to A is coalesced, and transposed through LDS.
-template <class T>
-__global__ __launch_bounds__(64) void k_chol_trsm(T *__restrict__ A, long lda, int j0, int jb, int r0,
-                                                  int n, const T *__restrict__ divisor) {
-#pragma clang fp contract(off)
-    __shared__ T L[CNB][CNB + 1];
     __shared__ T X[64][CNB + 1];
     __shared__ T sdiv[CNB];
-    const int t = threadIdx.x;
-    const int rb = r0 + blockIdx.x * 64;  // first row of this workgroup
-    const int nr = n - rb < 64 ? n - rb : 64;
-    {   // all loads in flight before their LDS stores (L11, divisors, rows)
-        T v[CNB];
+    const int tid = threadIdx.x, t = tid & 63;
+    const bool dw = tid < 64;  // wave 0: diagonal block; wave 1: rows below
+    const int rb = r0 + (int)blockIdx.x * 64;
+    const int nr = n - rb >= 64 ? 64 : (n - rb > 0 ? n - rb : 0);
+    {   // all loads in flight before their LDS stores
+        T v[64];
+        if (dw) {
 #pragma unroll
-        for (int r = 0; r < CNB; ++r) v[r] = (r < jb && t <= r) ? A[(long)(j0 + r) * lda + j0 + t] : T(0);
-        T x[64];
+            for (int r = 0; r < CNB; ++r) v[r] = (r < jb && t <= r) ? A[(long)(j0 + r) * lda + j0 + t] : T(0);
 #pragma unroll
-        for (int r = 0; r < 64; ++r) x[r] = (r < nr && t < jb) ? A[(long)(rb + r) * lda + j0 + t] : T(0);
-        const T dv = t < jb ? divisor[j0 + t] : T(1);
+            for (int r = 0; r < CNB; ++r)
+                if (r < jb && t <= r) a[r][t] = v[r];
+        } else {
 #pragma unroll
-        for (int r = 0; r < CNB; ++r)
-            if (r < jb && t <= r) L[r][t] = v[r];
+            for (int r = 0; r < 64; ++r) v[r] = (r < nr && t < jb) ? A[(long)(rb + r) * lda + j0 + t] : T(0);
 #pragma unroll
-        for (int r = 0; r < 64; ++r) X[r][t] = x[r];
-        sdiv[t] = dv;
+            for (int r = 0; r < 64; ++r) X[r][t] = v[r];
+        }
     }
     __syncthreads();
-    T x[CNB];  // this thread's row
+    T x[CNB];  // wave 0: row t of the diagonal block; wave 1: row rb + t
 #pragma unroll
-    for (int c = 0; c < CNB; ++c) x[c] = X[t][c];
-    static_for<CNB>([&](auto cc) __attribute__((always_inline)) {
-        constexpr int c = decltype(cc)::value;
-        if (c < jb) x[c] = (x[c] - (c ? dot8_reg<c>(x, &L[c][0]) : T(0))) / sdiv[c];
+    for (int c = 0; c < CNB; ++c) x[c] = dw ? (c <= t ? a[t][c] : T(0)) : X[t][c];
+    bool failed = false;
+    int jend = jb;  // columns whose divisor was computed
+    static_for<CNB + 1>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        // wave 1: column j-1 of its rows (row j-1 of L11 and divisor j-1 are final)
+        if constexpr (j > 0) {
+            if (!dw && j - 1 < jb && !failed)
+                x[j - 1] = (x[j - 1] - (j > 1 ? dot8_reg<j - 1>(x, &a[j - 1][0]) : T(0))) / sdiv[j - 1];
+        }
+        if constexpr (j < CNB) {
+            if (j < jb && !failed) {  // the reference stops at the first failure
+                if (dw && j > 0 && t >= j && t < jb) {
+                    x[j] = x[j] - dot8_reg<j>(x, &a[j][0]);
+                    a[t][j] = x[j];
+                }
+                __syncthreads();
+                const T diag = a[j][j];
+                if (fabs(diag) < ceps<T>() || diag < T(0)) {
+                    if (tid == 0 && blockIdx.x == 0 && *info == (int64_t)n) *info = j0 + j;
+                    failed = true;
+                    jend = j + 1;
+                }
+                const T div = sqrt(diag);
+                if (dw && t >= j && t < jb) {
+                    x[j] = x[j] / div;
+                    a[t][j] = x[j];
+                }
+                // kept in LDS: a global store here would make the barrier wait for it
+                if (tid == 0) sdiv[j] = div;
+                __syncthreads();
+            }
+        }
     });
+    if (dw) {
+        if (blockIdx.x == 0) {
+            if (t < jend) divisor[j0 + t] = sdiv[t];
+#pragma unroll 8
+            for (int r = 0; r < jb; ++r)
+                if (t <= r) A[(long)(j0 + r) * lda + j0 + t] = a[r][t];
+        }
+    } else {
 #pragma unroll
-    for (int c = 0; c < CNB; ++c) X[t][c] = x[c];
+        for (int c = 0; c < CNB; ++c) X[t][c] = x[c];
+    }
     __syncthreads();
-    if (t < jb) {
+    if (!dw && t < jb) {
 #pragma unroll 8
         for (int r = 0; r < nr; ++r) A[(long)(rb + r) * lda + j0 + t] = X[r][t];
     }
@@ -179,17 +177,12 @@ static int chol_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info, v
         const size_t kb = n - k0 < (size_t)CONB ? n - k0 : CONB;
         for (size_t j0 = k0; j0 < k0 + kb; j0 += CNB) {
             const size_t jb = k0 + kb - j0 < (size_t)CNB ? k0 + kb - j0 : CNB;
-            {
-                RLG_PROF("chol_diag", s);
-                hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(64), 0, s, a, (long)lda, (int)j0,
-                                   (int)jb, divisor, info, (int)n);
-                RLG_LAUNCHED();
-            }
             const size_t r0 = j0 + jb;
-            if (r0 < n) {
-                RLG_PROF("chol_trsm", s);
-                hipLaunchKernelGGL(k_chol_trsm<T>, dim3(ceil_div(n - r0, 64)), dim3(64), 0, s, a,
-                                   (long)lda, (int)j0, (int)jb, (int)r0, (int)n, divisor);
+            {
+                RLG_PROF("chol_block", s);
+                const size_t below = n - r0;  // rows under the diagonal block
+                hipLaunchKernelGGL(k_chol_block<T>, dim3(below ? ceil_div(below, 64) : 1), dim3(128), 0,
+                                   s, a, (long)lda, (int)j0, (int)jb, (int)r0, (int)n, divisor, info);
                 RLG_LAUNCHED();
             }
             if (r0 < k0 + kb) {
