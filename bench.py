@@ -83,7 +83,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU layout is described in DESIGN.md, not built)", file=sys.stderr)
+              "(one process drives one GPU; a multi-GPU layout is not built)", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)

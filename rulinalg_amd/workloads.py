@@ -1,24 +1,24 @@
-"""Seeded synthetic inputs of the benchmark configurations.
+"""Seeded synthetic inputs of the benchmark workloads.
 
 One generator for everything: numpy's Philox4x32-10 bit generator
 (Salmon et al., SC'11) seeded with a small integer, so the CPU oracle and the
 GPU see bit-identical inputs.  uniform(-1, 1) and standard normal draws use
 numpy's Generator.uniform / Generator.standard_normal on that bit generator.
 
-  config 1 (n = 256):  GEMM A ~ U[-1,1] seed 0, B ~ U[-1,1] seed 1;
+  order 256:          GEMM A ~ U[-1,1] seed 0, B ~ U[-1,1] seed 1;
                        LU A ~ N(0,1) seed 2, b ~ N(0,1) seed 3
-  config 2:            DGEMM 8192^3 U[-1,1] seeds 10, 11; LU 16384 N(0,1)
+  order 8192 / 16384: DGEMM 8192^3 U[-1,1] seeds 10, 11; LU 16384 N(0,1)
                        seed 12 with 16 RHS seed 13; Cholesky 16384
                        A = G G^T / n + I, G ~ N(0,1) seed 14
-  config 3:            A = U diag(s) V^T with s log-spaced on [1, kappa],
+  mixed, order 16384: A = U diag(s) V^T with s log-spaced on [1, kappa],
                        U, V orthogonal (seeds 20, 21), x_true seed 22
 """
 import numpy as np
 
-CONFIG1 = {"n": 256, "gemm_a": 0, "gemm_b": 1, "lu_a": 2, "lu_b": 3}
-CONFIG2 = {"gemm_n": 8192, "gemm_a": 10, "gemm_b": 11, "lu_n": 16384, "lu_a": 12, "lu_b": 13,
-           "lu_nrhs": 16, "chol_n": 16384, "chol_g": 14}
-CONFIG3 = {"n": 16384, "u": 20, "v": 21, "x": 22, "kappa": 1e4, "kappa_fallback": 1e8}
+ORDER256 = {"n": 256, "gemm_a": 0, "gemm_b": 1, "lu_a": 2, "lu_b": 3}
+ORDER16384 = {"gemm_n": 8192, "gemm_a": 10, "gemm_b": 11, "lu_n": 16384, "lu_a": 12, "lu_b": 13,
+              "lu_nrhs": 16, "chol_n": 16384, "chol_g": 14}
+MIXED16384 = {"n": 16384, "u": 20, "v": 21, "x": 22, "kappa": 1e4, "kappa_fallback": 1e8}
 
 
 def rng(seed):

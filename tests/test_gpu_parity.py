@@ -325,10 +325,10 @@ def test_lu_det_inverse_vs_oracle(n):
     assert np.abs(inv @ a - np.eye(n)).max() < 1e-11
 
 
-def test_config1_parity():
-    """BASELINE config 1 on the GPU: GEMM 256^3 U[-1,1] (seeds 0, 1), LU of a
-    256x256 N(0,1) (seed 2), one RHS (seed 3)."""
-    c1 = wl.CONFIG1
+def test_order256_parity():
+    """The order-256 workload on the GPU: GEMM 256^3 U[-1,1] (seeds 0, 1), LU
+    of a 256x256 N(0,1) (seed 2), one RHS (seed 3)."""
+    c1 = wl.ORDER256
     n = c1["n"]
     a = wl.uniform(n, n, c1["gemm_a"])
     b = wl.uniform(n, n, c1["gemm_b"])
