@@ -425,11 +425,26 @@ def batched(lib, torch, dev, count=20000, nmin=8, nmax=512, seed=30):
     assert lib.rlg_batch_autotune(64, rep, len(rep)) == 0, _lib.last_error()
     out["autotune"] = [line.split(";")[0] for line in rep.value.decode().splitlines()]
     log(f"batched: autotuned in {time.time() - t0:.1f}s: {out['autotune']}")
+    graph = C.c_void_p()
+    assert lib.rlg_batch_graph_create(count, it_dev.data_ptr(), C.byref(graph)) == 0, _lib.last_error()
+
+    def run_graph():
+        A.copy_(A0)
+        B.copy_(B0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        assert lib.rlg_batch_graph_launch(graph, None) == 0, _lib.last_error()
+        assert lib.rlg_synchronize() == 0
+        return (time.perf_counter() - t0) * 1e3, 0
+
     for mixed in (False, True):
-        run(mixed)  # warm
+        # plain: the captured graph (counter reset + persistent kernel)
+        # replayed; mixed: the eager launch (it has no graph form)
+        fn = (lambda: run(True)) if mixed else run_graph
+        fn()  # warm
         times = []
         for _ in range(3):
-            ms, fb = run(mixed)
+            ms, fb = fn()
             times.append(ms)
         # statuses and backward errors of a sample (host check)
         it_back = (_lib.BatchItem * count)()
@@ -447,11 +462,16 @@ def batched(lib, torch, dev, count=20000, nmin=8, nmax=512, seed=30):
         best = min(times)
         key = "batched_mixed" if mixed else "batched_plain"
         out[key] = {"ms": round(best, 3), "solves_per_s": round(count / (best / 1e3), 1),
+                    "launch": "eager" if mixed else "hipGraph replay",
                     "gflops": round(work / (best / 1e3) / 1e9, 1),
                     "frac_fp64_peak": round(work / (best / 1e3) / 78.6e12, 4),
                     "failed_items": bad, "fallbacks": fb, "backward_error_max_sample": max(etas)}
         log(f"{key}: {best:.1f} ms, {count / (best / 1e3):.0f} solves/s, eta max {max(etas):.1e}")
 
+    lib.rlg_batch_graph_destroy(graph)
+    # eager plain launch, for comparison with the graph replay
+    run(False)
+    out["batched_plain"]["eager_ms"] = round(min(run(False)[0] for _ in range(2)), 3)
     # The same kind of systems one at a time through the plain per-system
     # path (decompose + solve on the device, the drop-in's entry points), on
     # a sample: what the batched launch is reported against.
@@ -598,11 +618,16 @@ def cholesky_16384(lib, torch, dev, sptr):
     S = torch.empty_like(G)
     Gt = G.t().contiguous()
     torch.cuda.synchronize()
-    lib.rlg_gemm_f64_dev(n, n, n, 1.0 / n, G.data_ptr(), n, Gt.data_ptr(), n, 0.0, S.data_ptr(), n, sptr)
+    st = lib.rlg_gemm_f64_dev(n, n, n, 1.0 / n, G.data_ptr(), n, Gt.data_ptr(), n, 0.0, S.data_ptr(), n,
+                              sptr)
+    assert st == 0, _lib.last_error()
     assert lib.rlg_synchronize() == 0
     del G, Gt
     S.diagonal().add_(1.0)
     W = torch.empty_like(S)
+    g.manual_seed(15)
+    b = torch.randn(1, n, dtype=torch.float64, device=dev, generator=g)
+    x = torch.empty_like(b)
     torch.cuda.synchronize()
     ct = []
     for it in range(2):
@@ -614,30 +639,84 @@ def cholesky_16384(lib, torch, dev, sptr):
         assert st == 0, _lib.last_error()
         assert lib.rlg_synchronize() == 0
         ct.append((time.perf_counter() - t0) * 1e3)
+        if it == 1:  # one solve from the timed factor, outside the clock
+            assert lib.rlg_chol_solve_f64_dev(h, 1, b.data_ptr(), n, x.data_ptr(), n, sptr) == 0
+            assert lib.rlg_synchronize() == 0
         lib.rlg_chol_free(h)
+    eta = C.c_double()
+    assert lib.rlg_backward_error_f64_dev(n, S.data_ptr(), n, 1, b.data_ptr(), n, x.data_ptr(), n,
+                                          C.byref(eta), sptr) == 0, _lib.last_error()
+    # factor probe: ||S z - L (L^T z)||inf / (||S||inf ||z||inf)
+    z = torch.randn(n, 4, dtype=torch.float64, device=dev, generator=g)
+    low = torch.tril(W)
+    probe = ((S @ z - low @ (low.t() @ z)).abs().max() /
+             (S.abs().sum(1).max() * z.abs().max())).item()
+    del low
+    assert eta.value <= 1e-13, f"Cholesky 16384 backward error {eta.value:.3e}"
+    assert probe <= 1e-13, f"Cholesky 16384 factor residual {probe:.3e}"
     tf = n ** 3 / 3 / (min(ct) / 1e3) / 1e12
-    log(f"Cholesky 16384: {min(ct):.1f} ms, {tf:.1f} TF/s")
-    return {"ms": round(min(ct), 3), "tflops": round(tf, 2), "frac_fp64_peak": round(tf / 78.6, 4)}
+    log(f"Cholesky 16384: {min(ct):.1f} ms, {tf:.1f} TF/s, eta {eta.value:.2e}, factor residual {probe:.2e}")
+    return {"ms": round(min(ct), 3), "tflops": round(tf, 2), "frac_fp64_peak": round(tf / 78.6, 4),
+            "backward_error": eta.value, "factor_residual": probe}
+
+
+class one_core:
+    """Pins the calling thread (and so the single-threaded oracle) to one core
+    for the duration, in process."""
+
+    def __enter__(self):
+        self.saved = os.sched_getaffinity(0)
+        self.core = min(self.saved)
+        os.sched_setaffinity(0, {self.core})
+        return self
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.saved)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def cpu_baseline(cpu_n, n):
     """rulinalg's PartialPivLu::decompose + solve as restated by the oracle
-    (bit-exact with the Rust code), single core, on a bounded sample."""
+    (bit-exact with the Rust code), pinned to one core, on a bounded sample:
+    one solve at order cpu_n, measured, then scaled by (n / cpu_n)^3.  Also
+    the oracle's blocked single-thread DGEMM (the stand-in for
+    matrixmultiply::dgemm, which the reference's `&a * &b` calls) at order
+    1024, measured."""
     from oracle import oracle as orc
     from rulinalg_amd import workloads as wl
     # the benchmark's matrix kind at a smaller order: U diag(s) V^T, kappa 1e4
     a = np.ascontiguousarray(wl.conditioned(cpu_n, 1e4, 12, 13))
     b = wl.gaussian(cpu_n, 1, 14)[:, 0]
-    t0 = time.perf_counter()
-    lu, perm, _ = orc.lu_decompose(a)
-    orc.lu_solve(lu, perm, b)
-    dt = time.perf_counter() - t0
+    ga = wl.uniform(1024, 1024, 10)
+    gb = wl.uniform(1024, 1024, 11)
+    with one_core() as pin:
+        t0 = time.perf_counter()
+        lu, perm, _ = orc.lu_decompose(a)
+        orc.lu_solve(lu, perm, b)
+        dt = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.gemm_blocked(ga, gb)
+        dg = time.perf_counter() - t0
     scale = (n / cpu_n) ** 3
-    log(f"CPU oracle LU n={cpu_n}: {dt:.2f}s -> {dt * scale:.0f}s at n={n}")
+    log(f"CPU oracle LU n={cpu_n} on core {pin.core}: {dt:.2f}s -> {dt * scale:.0f}s at n={n}; "
+        f"DGEMM 1024: {dg:.2f}s")
     return {"value": 1.0 / (dt * scale), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"PartialPivLu::decompose + solve, n={cpu_n} U diag(s) V^T kappa 1e4, {dt:.2f}s, "
-                      f"scaled by (n/{cpu_n})^3 to n={n}",
-            "host": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+            "sample": f"PartialPivLu::decompose + solve, n={cpu_n} U diag(s) V^T kappa 1e4, "
+                      f"measured {dt:.2f}s on one pinned core, scaled by (n/{cpu_n})^3 to n={n}",
+            "measured_s_at_sample": round(dt, 3), "sample_n": cpu_n,
+            "dgemm_1024": {"s": round(dg, 3), "gflops": round(2 * 1024 ** 3 / dg / 1e9, 2),
+                           "what": "oracle's packed blocked single-thread DGEMM (stand-in for "
+                                   "matrixmultiply::dgemm), one pinned core"},
+            "host": cpu_model(), "pinned_core": pin.core, "nproc": os.cpu_count()}
 
 
 if __name__ == "__main__":

@@ -22,7 +22,9 @@
  *   dot, lu_decompose, lu_solve, forward/back substitutions, chol_decompose,
  *   chol_solve, transpose_back_substitution, permutation helpers, matvec,
  *   det/inverse: the reference's known-answer tests (tests/golden/kats.json),
- *   LAPACK's dgetrf pivot sequence and factors (tests/golden/lapack_lu_*.npz),
+ *   LAPACK dgetrf / dpotrf run on the same inputs through scipy.linalg
+ *   (lu_factor, cholesky: tests/test_oracle.py::test_lu_matches_lapack,
+ *   test_chol_matches_lapack),
  *   reconstruction and backward-error identities, and brute force on tiny
  *   cases (tests/test_oracle.py).
  *   matmul: the reference's exact small products (mat_mul.rs:293-427) and a

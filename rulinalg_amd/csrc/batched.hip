@@ -640,7 +640,7 @@ __global__ void k_fill_u32_graph(unsigned *p, unsigned count) {
     if (threadIdx.x < count) p[threadIdx.x] = 0u;
 }
 
-__global__ __launch_bounds__(BTH, 2) void k_batch_f64(rlg_batch_item *__restrict__ items,
+__global__ __launch_bounds__(BTH) void k_batch_f64(rlg_batch_item *__restrict__ items,
                                                    const int *__restrict__ order,
                                                    const unsigned char *__restrict__ nbs, int count,
                                                    unsigned *__restrict__ counter) {
@@ -927,7 +927,10 @@ int nb_for(int n) {
     return nb;
 }
 
-int resident_grid(Device *d) { return 2 * d->num_cus; }
+// One workgroup per CU: the kernels take 256 VGPRs + 256 AGPRs per lane
+// (one wave per SIMD) and 71 KB of LDS, so a second workgroup would only
+// wait for the first to finish.
+int resident_grid(Device *d) { return d->num_cus; }
 
 struct Prepared {
     PoolBuf order, nbs, counter;

@@ -14,16 +14,18 @@ struct rlg_lu {
     int elem = 8;           // 8: f64 factor, 4: f32 factor
     size_t n = 0, lda = 0;
     void *a = nullptr;      // packed factor on the device
-    DevBuf own;             // backing store when the handle owns `a`
-    DevBuf raw_d;           // raw permutation (int64) on the device
+    PoolBuf own;            // backing store when the handle owns `a`
+    PoolBuf raw_d;          // raw permutation (int64) on the device
     std::vector<int64_t> ipiv, raw;
+    mutable StreamUse use;  // streams of asynchronous solves on this factor
 };
 
 struct rlg_chol {
     int elem = 8;
     size_t n = 0, lda = 0;
     void *a = nullptr;
-    DevBuf own;
+    PoolBuf own;
+    mutable StreamUse use;
 };
 
 namespace {
@@ -113,7 +115,7 @@ int lu_finish(hipStream_t s, rlg_lu *h, int64_t *ipiv_d, int64_t *info_d, void *
         return RLG_E_DIV_BY_ZERO;
     }
     perm_from_ipiv(n, h->ipiv.data(), h->raw.data());
-    RLG_TRY(h->raw_d.alloc(n * 8));
+    RLG_TRY(h->raw_d.alloc(n * 8, s));
     if (n) RLG_HIP(hipMemcpyAsync(h->raw_d.p, h->raw.data(), n * 8, hipMemcpyHostToDevice, s));
     RLG_HIP(hipStreamSynchronize(s));
     return RLG_OK;
@@ -147,7 +149,7 @@ int lu_decompose_host(size_t rows, size_t cols, const T *a, size_t lda, rlg_lu *
     h->elem = sizeof(T);
     h->n = rows;
     h->lda = rows;
-    int st = h->own.alloc(rows * rows * sizeof(T));
+    int st = h->own.alloc(rows * rows * sizeof(T), s);
     if (st == RLG_OK) {
         h->a = h->own.p;
         st = upload_matrix<T>(s, static_cast<T *>(h->a), rows, a, lda, rows, rows);
@@ -218,7 +220,7 @@ int chol_decompose_host(size_t rows, size_t cols, const T *a, size_t lda, rlg_ch
     h->elem = sizeof(T);
     h->n = rows;
     h->lda = rows;
-    int st = h->own.alloc(rows * rows * sizeof(T));
+    int st = h->own.alloc(rows * rows * sizeof(T), s);
     if (st == RLG_OK) {
         h->a = h->own.p;
         st = upload_matrix<T>(s, static_cast<T *>(h->a), rows, a, lda, rows, rows);
@@ -474,6 +476,7 @@ int rlg_lu_solve_f64_dev(const rlg_lu *lu, size_t nrhs, const double *b, size_t 
     hipStream_t s = pick_stream(d, stream);
     PoolBuf tmp;  // returned to the pool with an event: the call stays asynchronous
     RLG_TRY(tmp.alloc(nrhs * lu->n * 8, s));
+    lu->use.add(s);
     return lu_solve<double>(s, lu->n, static_cast<const double *>(lu->a), lu->lda,
                             lu->raw_d.as<int64_t>(), nrhs, b, ldb, x, ldx, tmp.as<double>());
 }
@@ -541,9 +544,10 @@ int rlg_lu_det_f64(const rlg_lu *lu, double *det) {
     std::vector<double> diag(n);
     Device *d;
     RLG_TRY(current_device(&d));
-    for (size_t i = 0; i < n; ++i)
-        RLG_HIP(hipMemcpyAsync(&diag[i], static_cast<const double *>(lu->a) + i * lu->lda + i, 8,
-                               hipMemcpyDeviceToHost, d->stream));
+    // the diagonal as one strided copy (row pitch lda + 1)
+    if (n)
+        RLG_HIP(hipMemcpy2DAsync(diag.data(), 8, lu->a, (lu->lda + 1) * 8, 8, n,
+                                 hipMemcpyDeviceToHost, d->stream));
     RLG_HIP(hipStreamSynchronize(d->stream));
     // lu.rs:291-300: fold the diagonal from one, times the parity of p.
     double u_det = 1.0;
@@ -570,8 +574,12 @@ int rlg_lu_inverse_f64(const rlg_lu *lu, double *inv, size_t ld) {
 }
 
 void rlg_lu_free(rlg_lu *lu) {
-    // asynchronous _dev solves may still read the factor and permutation
-    if (lu) (void)hipDeviceSynchronize();
+    if (!lu) return;
+    // asynchronous _dev solves may still read the factor and permutation: the
+    // memory goes back to the pool behind events on the streams they ran on
+    const auto streams = lu->use.all();
+    lu->own.release_after(streams);
+    lu->raw_d.release_after(streams);
     delete lu;
 }
 
@@ -622,6 +630,7 @@ int rlg_chol_solve_f64_dev(const rlg_chol *ch, size_t nrhs, const double *b, siz
     hipStream_t s = pick_stream(d, stream);
     PoolBuf tmp;  // returned to the pool with an event: the call stays asynchronous
     RLG_TRY(tmp.alloc(nrhs * ch->n * 8, s));
+    ch->use.add(s);
     return chol_solve<double>(s, ch->n, static_cast<const double *>(ch->a), ch->lda, nrhs, b, ldb, x,
                               ldx, tmp.as<double>());
 }
@@ -641,9 +650,9 @@ int rlg_chol_det_f64(const rlg_chol *ch, double *det) {
     std::vector<double> diag(n);
     Device *d;
     RLG_TRY(current_device(&d));
-    for (size_t i = 0; i < n; ++i)
-        RLG_HIP(hipMemcpyAsync(&diag[i], static_cast<const double *>(ch->a) + i * ch->lda + i, 8,
-                               hipMemcpyDeviceToHost, d->stream));
+    if (n)
+        RLG_HIP(hipMemcpy2DAsync(diag.data(), 8, ch->a, (ch->lda + 1) * 8, 8, n,
+                                 hipMemcpyDeviceToHost, d->stream));
     RLG_HIP(hipStreamSynchronize(d->stream));
     double l_det = 1.0;  // cholesky.rs:175-180
     for (size_t i = 0; i < n; ++i) l_det = l_det * diag[i];
@@ -664,7 +673,8 @@ int rlg_chol_inverse_f64(const rlg_chol *ch, double *inv, size_t ld) {
 }
 
 void rlg_chol_free(rlg_chol *ch) {
-    if (ch) (void)hipDeviceSynchronize();
+    if (!ch) return;
+    ch->own.release_after(ch->use.all());
     delete ch;
 }
 

@@ -138,9 +138,12 @@ __global__ __launch_bounds__(128) void k_chol_block(T *__restrict__ A, long lda,
                     jend = j + 1;
                 }
                 const T div = sqrt(diag);
+                // the divided diagonal stays in thread j's register until the
+                // end: every thread of both waves reads the undivided a[j][j]
+                // between the two barriers, so nothing may overwrite it there
                 if (dw && t >= j && t < jb) {
                     x[j] = x[j] / div;
-                    a[t][j] = x[j];
+                    if (t != j) a[t][j] = x[j];
                 }
                 // kept in LDS: a global store here would make the barrier wait for it
                 if (tid == 0) sdiv[j] = div;
@@ -149,6 +152,9 @@ __global__ __launch_bounds__(128) void k_chol_block(T *__restrict__ A, long lda,
         }
     });
     if (dw) {
+#pragma unroll
+        for (int c = 0; c < CNB; ++c)
+            if (c == t && c < jend) a[t][t] = x[c];
         if (blockIdx.x == 0) {
             if (t < jend) divisor[j0 + t] = sdiv[t];
 #pragma unroll 8

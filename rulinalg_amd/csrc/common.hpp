@@ -105,7 +105,28 @@ struct PoolBuf {
     ~PoolBuf() { release(); }
     int alloc(size_t n, hipStream_t s);
     void release();
+    // Back to the pool once every stream in `streams` (and the buffer's own
+    // stream) has reached the point where this is called; nothing waits on
+    // the host.
+    void release_after(const std::vector<hipStream_t> &streams);
     template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// The streams that ran work reading a factor handle's device memory, so that
+// freeing the handle waits for exactly those streams.
+struct StreamUse {
+    std::mutex mu;
+    std::vector<hipStream_t> streams;
+    void add(hipStream_t s) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto x : streams)
+            if (x == s) return;
+        streams.push_back(s);
+    }
+    std::vector<hipStream_t> all() {
+        std::lock_guard<std::mutex> lk(mu);
+        return streams;
+    }
 };
 // Frees every pooled buffer that is not in use (all devices).
 void pool_trim();

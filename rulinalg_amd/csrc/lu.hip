@@ -4,9 +4,10 @@
 // Blocking: outer blocks of ONB = 512 columns drive the trailing Schur
 // update as one MFMA GEMM of depth 512 (faster than 256 at order 16384 in
 // both FP64 and FP32: less C traffic per flop); inside an outer block, inner panels
-// of PNB = 64 columns are factored by a cooperative kernel and applied to
-// the rest of the outer block by a 64-row triangular solve and a depth-64
-// GEMM.
+// of PNB = 64 columns are factored by a kernel launched cooperatively
+// (hipLaunchCooperativeKernel: all of its workgroups resident together) and
+// applied to the rest of the outer block by a 64-row triangular solve and a
+// depth-64 GEMM.
 //
 // The panel kernel keeps the whole (n - k) x 64 panel resident in LDS,
 // spread over G workgroups of PR rows each.  Every column costs one
@@ -762,13 +763,20 @@ static int panel_any(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t
 }
 
 // Largest panel height: the panel's workgroups exchange data while they run,
-// so all of them must be resident at once (one per CU), and their slots must
-// fit the exchange area.
+// so all of them must be resident at once, and their slots must fit the
+// exchange area.  The bound is the device's occupancy for this kernel (one
+// workgroup per CU: its LDS tile takes most of the CU's LDS) times its CUs.
 template <class T>
 static int panel_rows_max(size_t *out) {
     Device *d = nullptr;
     RLG_TRY(current_device(&d));
-    const size_t g = (size_t)(d->num_cus < GMAX ? d->num_cus : GMAX);
+    static thread_local int per_cu[2] = {-1, -1};
+    int &bpc = per_cu[sizeof(T) == 8];
+    if (bpc < 0)
+        RLG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &bpc, reinterpret_cast<const void *>(k_lu_panel<T>), PTHREADS, 0));
+    size_t g = (size_t)bpc * (size_t)d->num_cus;
+    if (g > (size_t)GMAX) g = GMAX;
     *out = g * PanelRows<T>::value;
     return RLG_OK;
 }
@@ -785,9 +793,17 @@ static int panel(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, 
     const unsigned G = ceil_div(m, PanelRows<T>::value);
     // algorithmic bytes: the m x w panel read once and written once
     RLG_PROF_W("lu_panel", s, 2.0 * (double)m * (double)w * sizeof(T));
-    hipLaunchKernelGGL(k_lu_panel<T>, dim3(G), dim3(PTHREADS), 0, s, a, (long)lda, (int)k, (int)m,
-                       (int)w, ipiv, info, (int)n, lw.xch, lw.timeout);
-    RLG_LAUNCHED();
+    // A cooperative launch: the runtime starts the grid only with every
+    // workgroup resident at once (and refuses a grid that cannot be), which
+    // the per-column exchange between workgroups relies on.
+    T *a_ = a;
+    long lda_ = (long)lda;
+    int k_ = (int)k, m_ = (int)m, w_ = (int)w, n_ = (int)n;
+    xword *xch_ = lw.xch;
+    int *to_ = lw.timeout;
+    void *args[] = {&a_, &lda_, &k_, &m_, &w_, &ipiv, &info, &n_, &xch_, &to_};
+    RLG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_lu_panel<T>), dim3(G),
+                                       dim3(PTHREADS), args, 0, s));
     return RLG_OK;
 }
 

@@ -149,7 +149,7 @@ struct PoolEntry {
     void *p;
     size_t bytes;
     int dev;
-    hipEvent_t ready;
+    std::vector<hipEvent_t> ready;  // every stream that used the buffer
 };
 std::mutex g_pool_mu;
 std::vector<PoolEntry> &pool_entries() {
@@ -176,9 +176,9 @@ int PoolBuf::alloc(size_t n, hipStream_t s) {
             v.erase(v.begin() + (long)best);
             p = e.p;
             bytes = e.bytes;
-            if (e.ready) {
-                (void)hipStreamWaitEvent(s, e.ready, 0);
-                (void)hipEventDestroy(e.ready);
+            for (hipEvent_t ev : e.ready) {
+                (void)hipStreamWaitEvent(s, ev, 0);
+                (void)hipEventDestroy(ev);
             }
             return RLG_OK;
         }
@@ -198,18 +198,28 @@ int PoolBuf::alloc(size_t n, hipStream_t s) {
     return RLG_OK;
 }
 
-void PoolBuf::release() {
-    if (!p) return;
+// An event recorded on s, or (if that fails) s drained on the host.
+static hipEvent_t mark(hipStream_t s) {
     hipEvent_t ev = nullptr;
     if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess) {
-        if (hipEventRecord(ev, stream) != hipSuccess) {
-            (void)hipEventDestroy(ev);
-            ev = nullptr;
-            (void)hipStreamSynchronize(stream);
-        }
+        if (hipEventRecord(ev, s) == hipSuccess) return ev;
+        (void)hipEventDestroy(ev);
     }
+    (void)hipStreamSynchronize(s);
+    return nullptr;
+}
+
+void PoolBuf::release() { release_after({}); }
+
+void PoolBuf::release_after(const std::vector<hipStream_t> &streams) {
+    if (!p) return;
+    std::vector<hipEvent_t> evs;
+    if (hipEvent_t ev = mark(stream)) evs.push_back(ev);
+    for (hipStream_t s : streams)
+        if (s != stream)
+            if (hipEvent_t ev = mark(s)) evs.push_back(ev);
     std::lock_guard<std::mutex> lk(g_pool_mu);
-    pool_entries().push_back({p, bytes, dev, ev});
+    pool_entries().push_back({p, bytes, dev, std::move(evs)});
     p = nullptr;
     bytes = 0;
 }
@@ -218,9 +228,9 @@ void pool_trim() {
     std::lock_guard<std::mutex> lk(g_pool_mu);
     auto &v = pool_entries();
     for (auto &e : v) {
-        if (e.ready) {
-            (void)hipEventSynchronize(e.ready);
-            (void)hipEventDestroy(e.ready);
+        for (hipEvent_t ev : e.ready) {
+            (void)hipEventSynchronize(ev);
+            (void)hipEventDestroy(ev);
         }
         (void)hipFree(e.p);
     }

@@ -25,6 +25,7 @@
 // size, so a result is bit-identical whether its request ran alone or in a
 // mix.
 #include <chrono>
+#include <cstdlib>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -398,9 +399,22 @@ void rlg_server_opts_default(rlg_server_opts *o) {
     o->tol = 1e-13;
 }
 
+// The request loop stays switched off until it has run on the GPU; the
+// environment variable lets its own tests reach it.
+static bool server_enabled() {
+    const char *e = getenv("RLG_ENABLE_SERVER");
+    return e && e[0] == '1';
+}
+
 int rlg_server_create(const rlg_server_opts *o, rlg_server **out) {
     if (!out) { set_error("null output"); return RLG_E_INVALID_ARG; }
     *out = nullptr;
+    if (!server_enabled()) {
+        set_error("rlg_server_* is disabled in this build: an earlier form of this request loop "
+                  "stalled on the GPU and this form has not been validated there; use the "
+                  "per-system and batched entry points (or rulinalg_amd.serving)");
+        return RLG_E_INVALID_ARG;
+    }
     Device *d;
     RLG_TRY(current_device(&d));
     auto *s = new rlg_server();

@@ -1,0 +1,295 @@
+"""GPU: the benchmark configurations at their stated sizes, checked through
+properties that do not need a CPU factorization of the whole matrix, plus the
+low-precision configurations of the mixed solve.
+
+Inputs are generated on the device with PyTorch; every factorization and
+solve goes through librlgpu's C ABI.  Tolerances (stated here, used below):
+  * solves: normwise backward error max_r ||b - A x||inf / (||A||inf ||x||inf
+    + ||b||inf) <= 1e-13, measured in FP64 on the device;
+  * factors: ||P A z - L (U z)||inf / (||A||inf ||z||inf) <= 1e-13 for a
+    random z (LU), ||S z - L (L^T z)||inf / (||S||inf ||z||inf) <= 1e-13
+    (Cholesky): a wrong row, a wrong permutation or a dropped update shows as
+    an O(1) residual;
+  * GEMM (Freivalds): ||C z - A (B z)||inf / (||A||inf ||B||inf ||z||inf)
+    <= 1e-13;
+  * batched: every item's status is RLG_OK, and a sample of 200 items matches
+    the oracle (backward error <= 1e-13; forward difference <= max(1e-13,
+    16 n kappa u)).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import rulinalg_amd as rl
+from oracle import oracle as orc
+from rulinalg_amd import _lib, workloads as wl
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+U = np.finfo(np.float64).eps
+
+
+@pytest.fixture(scope="module", autouse=True)
+def lib():
+    lib = _lib.load()
+    assert lib.rlg_set_device(0) == 0, _lib.last_error()
+    return lib
+
+
+def dev():
+    return torch.device("cuda:0")
+
+
+def gen(seed):
+    g = torch.Generator(device=dev())
+    g.manual_seed(seed)
+    return g
+
+
+def inf_norm(a):
+    return a.abs().sum(1).max().item()
+
+
+def eta_dev(lib, a, b, x):
+    n = a.shape[0]
+    out = C.c_double()
+    assert lib.rlg_backward_error_f64_dev(n, a.data_ptr(), n, b.shape[0], b.data_ptr(), n, x.data_ptr(),
+                                          n, C.byref(out), None) == 0, _lib.last_error()
+    return out.value
+
+
+def householder_conditioned(n, kappa, seed):
+    """A = H_U diag(s) H_V, s log-spaced on [1, kappa], H_U and H_V products of
+    eight Householder reflectors each (exactly orthogonal: the singular values
+    of A are s)."""
+    g = gen(seed)
+    a = torch.zeros(n, n, dtype=torch.float64, device=dev())
+    a.diagonal().copy_(torch.logspace(0.0, float(np.log10(kappa)), n, dtype=torch.float64,
+                                      device=dev()))
+    for side in (0, 1):
+        for _ in range(8):
+            v = torch.randn(n, dtype=torch.float64, device=dev(), generator=g)
+            v /= v.norm()
+            if side == 0:
+                a.addr_(v, a.t().mv(v), alpha=-2.0)
+            else:
+                a.addr_(a.mv(v), v, alpha=-2.0)
+    torch.cuda.synchronize()
+    return a
+
+
+def test_dgemm_8192_freivalds(lib):
+    m = 8192
+    a = torch.rand(m, m, dtype=torch.float64, device=dev(), generator=gen(10)) * 2 - 1
+    b = torch.rand(m, m, dtype=torch.float64, device=dev(), generator=gen(11)) * 2 - 1
+    c = torch.empty_like(a)
+    torch.cuda.synchronize()
+    st = lib.rlg_gemm_f64_dev(m, m, m, 1.0, a.data_ptr(), m, b.data_ptr(), m, 0.0, c.data_ptr(), m, None)
+    assert st == 0, _lib.last_error()
+    assert lib.rlg_synchronize() == 0
+    z = torch.randn(m, 4, dtype=torch.float64, device=dev(), generator=gen(12))
+    r = (c @ z - a @ (b @ z)).abs().max().item()
+    assert r / (inf_norm(a) * inf_norm(b) * z.abs().max().item()) <= 1e-13
+
+
+def test_lu_16384_16rhs(lib):
+    n, nrhs = 16384, 16
+    a0 = torch.randn(n, n, dtype=torch.float64, device=dev(), generator=gen(12))
+    b = torch.randn(nrhs, n, dtype=torch.float64, device=dev(), generator=gen(13))
+    a = a0.clone()
+    x = torch.empty_like(b)
+    torch.cuda.synchronize()
+    h = C.c_void_p()
+    assert lib.rlg_lu_decompose_f64_dev(n, a.data_ptr(), n, None, C.byref(h)) == 0, _lib.last_error()
+    try:
+        assert lib.rlg_lu_solve_f64_dev(h, nrhs, b.data_ptr(), n, x.data_ptr(), n, None) == 0
+        assert lib.rlg_synchronize() == 0
+        assert eta_dev(lib, a0, b, x) <= 1e-13
+        perm = (C.c_size_t * n)()
+        assert lib.rlg_lu_perm(h, perm) == 0
+    finally:
+        lib.rlg_lu_free(h)
+    # stored p (lu.rs:190-193): row i of the factor is row raw[i] of A,
+    # with perm[raw[i]] = i
+    perm = np.frombuffer(perm, dtype=np.uint64).astype(np.int64)
+    raw = np.empty(n, dtype=np.int64)
+    raw[perm] = np.arange(n)
+    assert np.array_equal(np.sort(raw), np.arange(n))
+    z = torch.randn(n, 4, dtype=torch.float64, device=dev(), generator=gen(14))
+    pa_z = (a0 @ z)[torch.from_numpy(raw).to(dev())]
+    uz = torch.triu(a) @ z
+    luz = torch.tril(a, -1) @ uz + uz
+    r = (pa_z - luz).abs().max().item()
+    assert r / (inf_norm(a0) * z.abs().max().item()) <= 1e-13
+
+
+def test_cholesky_16384(lib):
+    n = 16384
+    g = torch.randn(n, n, dtype=torch.float64, device=dev(), generator=gen(14))
+    s = g @ g.t().contiguous() / n
+    del g
+    s.diagonal().add_(1.0)
+    a = s.clone()
+    torch.cuda.synchronize()
+    h = C.c_void_p()
+    assert lib.rlg_chol_decompose_f64_dev(n, a.data_ptr(), n, None, C.byref(h)) == 0, _lib.last_error()
+    b = torch.randn(1, n, dtype=torch.float64, device=dev(), generator=gen(15))
+    x = torch.empty_like(b)
+    try:
+        assert lib.rlg_chol_solve_f64_dev(h, 1, b.data_ptr(), n, x.data_ptr(), n, None) == 0
+        assert lib.rlg_synchronize() == 0
+    finally:
+        lib.rlg_chol_free(h)
+    assert eta_dev(lib, s, b, x) <= 1e-13
+    z = torch.randn(n, 4, dtype=torch.float64, device=dev(), generator=gen(16))
+    low = torch.tril(a)
+    r = (s @ z - low @ (low.t() @ z)).abs().max().item()
+    assert r / (inf_norm(s) * z.abs().max().item()) <= 1e-13
+
+
+@pytest.mark.parametrize("prec,nrhs", [("f32", 1), ("f32", 32), ("f16", 32)])
+def test_mixed_16384_kappa_1e4(lib, prec, nrhs):
+    n = 16384
+    a = householder_conditioned(n, 1e4, 20)
+    b = torch.randn(nrhs, n, dtype=torch.float64, device=dev(), generator=gen(22))
+    x = torch.empty_like(b)
+    torch.cuda.synchronize()
+    o = _lib.MixedOpts({"f32": _lib.PREC_F32, "f16": _lib.PREC_F16}[prec], 0, 1e-13, 30, 1)
+    rep = _lib.MixedReport()
+    st = lib.rlg_solve_mixed_f64_dev(n, a.data_ptr(), n, nrhs, b.data_ptr(), n, x.data_ptr(), n,
+                                     C.byref(o), C.byref(rep), None)
+    assert st == 0, _lib.last_error()
+    assert rep.converged == 1 and rep.fell_back == 0
+    assert eta_dev(lib, a, b, x) <= 1e-13
+
+
+def test_mixed_fallback_4096_stalls(lib):
+    """kappa = 1e11 at order 4096: kappa times FP32's unit roundoff is far above
+    one, so refinement from the FP32 factors cannot converge; the solver must
+    notice the stall and the FP64 factorization must deliver the answer."""
+    n = 4096
+    a = householder_conditioned(n, 1e11, 30)
+    b = torch.randn(1, n, dtype=torch.float64, device=dev(), generator=gen(31))
+    x = torch.empty_like(b)
+    torch.cuda.synchronize()
+    o = _lib.MixedOpts(_lib.PREC_F32, 0, 1e-13, 30, 1)
+    rep = _lib.MixedReport()
+    st = lib.rlg_solve_mixed_f64_dev(n, a.data_ptr(), n, 1, b.data_ptr(), n, x.data_ptr(), n,
+                                     C.byref(o), C.byref(rep), None)
+    assert st == 0, _lib.last_error()
+    assert rep.fell_back == 1 and rep.converged == 1
+    assert eta_dev(lib, a, b, x) <= 1e-13
+
+
+@pytest.mark.parametrize("prec", [_lib.PREC_F16, _lib.PREC_BF16])
+@pytest.mark.parametrize("nrhs", [1, 32])
+def test_mixed_lowp_kappa_1e4(prec, nrhs):
+    """FP16 / BF16 trailing updates at kappa = 1e4 on an order with several
+    depth-512 updates (2048 = four outer blocks): refinement must reach
+    backward error <= 1e-13 without the FP64 fallback."""
+    n = 2048
+    a = wl.conditioned(n, 1e4, 40, 41)
+    b = wl.gaussian(nrhs, n, 42)
+    x, rep = rl.solve_mixed(a, b if nrhs > 1 else b[0], rl.MixedOptions(factor_prec=prec))
+    x = np.asarray(x).reshape(nrhs, n)
+    assert rep["converged"] == 1 and rep["fell_back"] == 0, rep
+    for i in range(nrhs):
+        r = np.abs(b[i] - a @ x[i]).max()
+        assert r / (np.abs(a).sum(1).max() * np.abs(x[i]).max() + np.abs(b[i]).max()) <= 1e-13
+
+
+@pytest.mark.parametrize("n", [2048, 4100])
+def test_cholesky_multi_block(n):
+    """Orders spanning several 512-column outer blocks (4100: eight plus a
+    4-column remainder) against the oracle's factor (2048) or through the
+    residual property (4100)."""
+    a = wl.spd(n, 800 + n)
+    ch = rl.Cholesky.decompose(rl.Matrix(a))
+    l = ch.unpack().to_numpy()
+    if n <= 2048:
+        l_or = orc.chol_unpack(orc.chol_decompose(a))
+        assert np.abs(l - l_or).max() / np.abs(l_or).max() <= 1e-10
+    z = wl.gaussian(n, 3, 900 + n)
+    r = np.abs(a @ z - l @ (l.T @ z)).max()
+    assert r / (np.abs(a).sum(1).max() * np.abs(z).max()) <= 1e-13
+    b = wl.gaussian(n, 1, 950 + n)[:, 0]
+    x = ch.solve(rl.Vector(b)).data()
+    assert np.abs(b - a @ x).max() / (np.abs(a).sum(1).max() * np.abs(x).max() + np.abs(b).max()) <= 1e-13
+
+
+@pytest.mark.parametrize("col", [5, 30, 63])
+def test_cholesky_fails_inside_block_with_rows_below(col):
+    """A negative pivot inside the first 64-column block of an order-200
+    matrix (rows below the block in other workgroups): both waves of every
+    workgroup must stop at the same column, and the reference's error kind
+    comes back."""
+    n = 200
+    a = wl.spd(n, 77)
+    a[col, col] = -3.0
+    with pytest.raises(rl.Error) as e:
+        rl.Cholesky.decompose(rl.Matrix(a))
+    assert e.value.kind() == rl.ErrorKind.DecompFailure
+
+
+def test_batched_20000(lib):
+    """The batched configuration at its stated size: 20,000 systems, n uniform
+    on [8, 512], half LU (Gaussian) and half Cholesky (G G^T / n + I), 1-8
+    right-hand sides, one persistent launch."""
+    count = 20000
+    rng = np.random.default_rng(30)
+    ns = rng.integers(8, 513, count)
+    rs = rng.integers(1, 9, count)
+    kinds = np.arange(count) % 2
+    a_off = np.concatenate([[0], np.cumsum(ns.astype(np.int64) ** 2)])
+    b_off = np.concatenate([[0], np.cumsum(ns.astype(np.int64) * rs)])
+    p_off = np.concatenate([[0], np.cumsum(ns.astype(np.int64))])
+    A0 = torch.empty(int(a_off[-1]), dtype=torch.float64, device=dev())
+    B0 = torch.empty(int(b_off[-1]), dtype=torch.float64, device=dev())
+    g = gen(30)
+    for n in np.unique(ns):
+        for kind in (0, 1):
+            idx = np.nonzero((ns == n) & (kinds == kind))[0]
+            if len(idx) == 0:
+                continue
+            m = torch.randn(len(idx), n, n, dtype=torch.float64, device=dev(), generator=g)
+            if kind == 1:
+                m = torch.bmm(m, m.transpose(1, 2).contiguous()) / n + torch.eye(
+                    n, dtype=torch.float64, device=dev())
+            for q, i in enumerate(idx):
+                A0[int(a_off[i]):int(a_off[i + 1])].copy_(m[q].reshape(-1))
+    B0.normal_(generator=g)
+    A, B = A0.clone(), B0.clone()
+    P = torch.empty(int(p_off[-1]), dtype=torch.int64, device=dev())
+    items = (_lib.BatchItem * count)()
+    for i in range(count):
+        items[i] = _lib.BatchItem(int(kinds[i]), int(ns[i]), int(rs[i]), -1,
+                                  A.data_ptr() + 8 * int(a_off[i]), B.data_ptr() + 8 * int(b_off[i]),
+                                  P.data_ptr() + 8 * int(p_off[i]))
+    it_dev = torch.empty(C.sizeof(items), dtype=torch.uint8, device=dev())
+    torch.cuda.synchronize()
+    assert lib.rlg_dev_upload(it_dev.data_ptr(), C.addressof(items), C.sizeof(items)) == 0
+    assert lib.rlg_batch_solve_f64_dev(count, it_dev.data_ptr(), None) == 0, _lib.last_error()
+    assert lib.rlg_synchronize() == 0
+    back = (_lib.BatchItem * count)()
+    assert lib.rlg_dev_download(C.addressof(back), it_dev.data_ptr(), C.sizeof(back)) == 0
+    assert all(back[i].status == 0 for i in range(count))
+    for i in np.random.default_rng(31).choice(count, 200, replace=False):
+        n, r = int(ns[i]), int(rs[i])
+        a = A0[int(a_off[i]):int(a_off[i + 1])].view(n, n).cpu().numpy()
+        b = B0[int(b_off[i]):int(b_off[i + 1])].view(r, n).cpu().numpy()
+        x = B[int(b_off[i]):int(b_off[i + 1])].view(r, n).cpu().numpy()
+        k = np.linalg.cond(a, np.inf)
+        tol = max(1e-13, 16 * n * k * U)
+        if kinds[i] == 0:
+            lu_or, perm_or, _ = orc.lu_decompose(a)
+            solve = lambda v: orc.lu_solve(lu_or, perm_or, v)  # noqa: E731
+        else:
+            w = orc.chol_decompose(a)
+            solve = lambda v: orc.chol_solve(w, v)  # noqa: E731
+        for q in range(r):
+            xo = solve(b[q])
+            e = np.abs(b[q] - a @ x[q]).max() / (np.abs(a).sum(1).max() * np.abs(x[q]).max() +
+                                                 np.abs(b[q]).max())
+            assert e <= 1e-13
+            assert np.abs(x[q] - xo).max() / np.abs(xo).max() <= tol

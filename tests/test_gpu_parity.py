@@ -428,19 +428,18 @@ def test_chol_nonsquare_panics():
 
 @pytest.mark.parametrize("prec", [_lib.PREC_F32, _lib.PREC_F16, _lib.PREC_BF16])
 def test_mixed_refined_well_conditioned(prec):
-    n = 600
+    """kappa = 1e2 at order 1600 (three depth-512 low-precision trailing
+    updates, 1600 = 3 * 512 + 64): kappa * u is 1e-5 (FP32), 0.05 (FP16) and
+    0.4 (BF16), so refinement must converge without the FP64 fallback in every
+    precision, which also shows the low-precision updates are right."""
+    n = 1600
     a = wl.conditioned(n, 1e2, 1, 2)
     xt = wl.gaussian(n, 2, 3).T.copy()
     b = xt @ a.T
     x, rep = rl.solve_mixed(a, b, rl.MixedOptions(factor_prec=prec))
-    assert rep["converged"] == 1
+    assert rep["converged"] == 1 and rep["fell_back"] == 0, rep
     for i in range(2):
         assert eta(a, x[i], b[i]) <= 1e-13
-    # kappa * u is 1e-5 (FP32) and 0.05 (FP16): refinement must converge
-    # without the FP64 fallback, which also shows the low-precision trailing
-    # updates (orders 600 = 256 + 256 + 88 use two of them) are right
-    if prec != _lib.PREC_BF16:
-        assert rep["fell_back"] == 0
 
 
 @pytest.mark.parametrize("prec", [_lib.PREC_F32, _lib.PREC_F16])
