@@ -88,6 +88,32 @@ int rlg_gemm_f32_dev(size_t m, size_t k, size_t n, float alpha,
 int rlg_matvec_f64(size_t m, size_t n, const double *a, size_t lda,
                    size_t v_len, const double *v, double *y);
 
+/* ---------------------------------------------------------------- permutations
+ * PermutationMatrix * Matrix and Matrix * PermutationMatrix
+ * (src/matrix/impl_permutation_mul.rs:53-127 and 129-204), with the
+ * into-buffer semantics of permutation_matrix.rs:318-358: row i of `a`
+ * becomes row perm[i] of `out` (rows form), column j of `a` becomes column
+ * perm[j] of `out` (cols form).  perm holds the reference's Vec<usize>.
+ * perm_len != rows (rows form) or != cols (cols form) returns RLG_E_SHAPE
+ * (the asserts at permutation_matrix.rs:424-436); a perm that is not a
+ * permutation of 0..perm_len returns RLG_E_INVALID_PERMUTATION
+ * (validate_permutation, permutation_matrix.rs:438-467).  out must not
+ * overlap a. */
+int rlg_permute_rows_f64(size_t perm_len, const size_t *perm, size_t rows, size_t cols,
+                         const double *a, size_t lda, double *out, size_t ldo);
+int rlg_permute_rows_f32(size_t perm_len, const size_t *perm, size_t rows, size_t cols,
+                         const float *a, size_t lda, float *out, size_t ldo);
+int rlg_permute_cols_f64(size_t perm_len, const size_t *perm, size_t rows, size_t cols,
+                         const double *a, size_t lda, double *out, size_t ldo);
+int rlg_permute_cols_f32(size_t perm_len, const size_t *perm, size_t rows, size_t cols,
+                         const float *a, size_t lda, float *out, size_t ldo);
+/* Device forms (asynchronous on `stream`): perm is a device array of n
+ * int64 that must be a permutation (not checked). */
+int rlg_permute_rows_f64_dev(size_t n, const int64_t *perm, size_t cols, const double *a,
+                             size_t lda, double *out, size_t ldo, void *stream);
+int rlg_permute_cols_f64_dev(size_t rows, size_t n, const int64_t *perm, const double *a,
+                             size_t lda, double *out, size_t ldo, void *stream);
+
 /* ---------------------------------------------------------------- triangular
  * BaseMatrix::solve_u_triangular / solve_l_triangular
  * (src/matrix/base/mod.rs:1015-1024, 1058-1067) on nrhs right-hand sides

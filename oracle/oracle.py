@@ -222,6 +222,29 @@ def perm_matrix(perm):
     return m
 
 
+def permute_rows(perm, a):
+    """PermutationMatrix * Matrix through permute_rows_into_buffer
+    (src/matrix/permutation_matrix.rs:318-331): source row i is copied to
+    buffer row perm[i]."""
+    a = np.asarray(a)
+    out = np.zeros_like(a)
+    for i, t in enumerate(perm):
+        out[int(t), :] = a[i, :]
+    return out
+
+
+def permute_cols(perm, a):
+    """Matrix * PermutationMatrix through permute_cols_into_buffer
+    (src/matrix/permutation_matrix.rs:342-358): in every row, source element j
+    is copied to position perm[j]."""
+    a = np.asarray(a)
+    out = np.zeros_like(a)
+    for r in range(a.shape[0]):
+        for j, t in enumerate(perm):
+            out[r, int(t)] = a[r, j]
+    return out
+
+
 def perm_inverse(perm):
     """PermutationMatrix::inverse (src/matrix/permutation_matrix.rs:137-148)."""
     p = np.ascontiguousarray(perm, dtype=np.uintp)

@@ -128,6 +128,9 @@ class _Operand:
     def __mul__(self, other):
         if isinstance(other, _Operand):
             return self._gemm(other)
+        if isinstance(other, PermutationMatrix):
+            # M * P permutes columns: column j goes to perm[j] (impl_permutation_mul.rs:129-204)
+            return other._permute(self, False)
         if isinstance(other, Vector):
             m = self.to_numpy()
             y = np.empty(self._rows)
@@ -297,12 +300,27 @@ class PermutationMatrix:
         m[self._perm, np.arange(n)] = 1.0
         return Matrix(m)
 
+    def _permute(self, m, by_rows):
+        """Row (P * M) or column (M * P) permutation of a Matrix on the GPU
+        (rlg_permute_rows / rlg_permute_cols)."""
+        a = np.ascontiguousarray(m.to_numpy())
+        out = np.empty_like(a)
+        perm = np.ascontiguousarray(self._perm, dtype=np.uint64)
+        lib = _lib.load()
+        f32 = a.dtype == np.float32
+        if by_rows:
+            fn = lib.rlg_permute_rows_f32 if f32 else lib.rlg_permute_rows_f64
+        else:
+            fn = lib.rlg_permute_cols_f32 if f32 else lib.rlg_permute_cols_f64
+        ld = max(a.shape[1], 1)
+        _check(fn(len(perm), perm.ctypes.data, a.shape[0], a.shape[1], a.ctypes.data, ld,
+                  out.ctypes.data, ld))
+        return Matrix(out)
+
     def __mul__(self, other):
         # P * Matrix permutes rows: row i goes to perm[i] (impl_permutation_mul.rs:53-127)
-        if isinstance(other, Matrix):
-            out = np.empty_like(other.to_numpy())
-            out[self._perm] = other.to_numpy()
-            return Matrix(out)
+        if isinstance(other, (Matrix, MatrixSlice)):
+            return self._permute(other, True)
         if isinstance(other, Vector):
             out = np.empty_like(other.data())
             out[self._perm] = other.data()

@@ -98,6 +98,12 @@ SIGNATURES = {
     "rlg_lu_unpack_f64": (i32, [vp, vp, vp, vp]),
     "rlg_lu_det_f64": (i32, [vp, PD]),
     "rlg_lu_inverse_f64": (i32, [vp, vp, sz]),
+    "rlg_permute_rows_f64": (i32, [sz, vp, sz, sz, vp, sz, vp, sz]),
+    "rlg_permute_rows_f32": (i32, [sz, vp, sz, sz, vp, sz, vp, sz]),
+    "rlg_permute_cols_f64": (i32, [sz, vp, sz, sz, vp, sz, vp, sz]),
+    "rlg_permute_cols_f32": (i32, [sz, vp, sz, sz, vp, sz, vp, sz]),
+    "rlg_permute_rows_f64_dev": (i32, [sz, vp, sz, vp, sz, vp, sz, vp]),
+    "rlg_permute_cols_f64_dev": (i32, [sz, sz, vp, vp, sz, vp, sz, vp]),
     "rlg_lu_free": (None, [vp]),
     "rlg_chol_decompose_f64": (i32, [sz, sz, vp, sz, C.POINTER(vp)]),
     "rlg_chol_decompose_f32": (i32, [sz, sz, vp, sz, C.POINTER(vp)]),

@@ -20,6 +20,7 @@
 #include <utility>
 
 #include "kernels.hpp"
+#include "wave.hpp"
 
 namespace rlg {
 
@@ -29,18 +30,6 @@ constexpr int CONB = 512;
 template <class T> __device__ __forceinline__ T ceps();
 template <> __device__ __forceinline__ double ceps<double>() { return 2.220446049250313e-16; }
 template <> __device__ __forceinline__ float ceps<float>() { return 1.1920929e-7f; }
-
-// Calls f(std::integral_constant<int, I>) for I = 0 .. N-1 in order, so a
-// loop body sees its index as a compile-time constant and registers indexed
-// by it stay registers.
-template <class F, int... Is>
-__device__ __forceinline__ void static_for_impl(F &f, std::integer_sequence<int, Is...>) {
-    (f(std::integral_constant<int, Is>{}), ...);
-}
-template <int N, class F>
-__device__ __forceinline__ void static_for(F &&f) {
-    static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
 
 // utils::dot (src/utils.rs:20-51) of x[0..LEN) held in registers and
 // l[0..LEN) in LDS, LEN known at compile time: partial sums p0..p7 over

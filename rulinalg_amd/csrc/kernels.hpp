@@ -78,14 +78,16 @@ int lu_factor_lowp(hipStream_t s, int prec, int pivot, size_t n, float *a, size_
                    int64_t *ipiv, int64_t *info, void *work);
 
 // Tournament-pivoting FP32 panel (tslu.hip), used by lu_factor_lowp: the
-// workspace of tslu_work_bytes() is cleared by tslu_reset once per
-// factorization; `epoch` increases by one per panel.  Panels of up to
-// tslu_max_rows() rows.
-size_t tslu_work_bytes();
-size_t tslu_max_rows();
-int tslu_reset(hipStream_t s, void *work);
+// workspace of tslu_work_bytes(n) bytes (for panels of up to n rows) is
+// cleared by tslu_reset once per factorization.  tslu_panel factors the
+// panel A[k:k+m, k:k+w] in place, writes the swaps ipiv[k..k+w) (the
+// reference's sequential convention), the first tiny pivot into *info, and
+// L11^-1 of the pivot block (64 x 64, zero outside w x w) into linv.
+size_t tslu_work_bytes(size_t n);
+size_t tslu_max_rows(size_t n);
+int tslu_reset(hipStream_t s, void *work, size_t n);
 int tslu_panel(hipStream_t s, float *a, size_t lda, size_t k, size_t m, size_t w, int64_t *ipiv,
-               int64_t *info, size_t n, void *work, unsigned epoch, int *timeout);
+               int64_t *info, size_t n, void *work, float *linv);
 
 // Device flag raised by a panel barrier that gave up waiting (see lu.hip);
 // nonzero after lu_factor means the factorization is invalid.

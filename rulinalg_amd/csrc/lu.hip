@@ -743,21 +743,27 @@ __global__ __launch_bounds__(256) void k_trsm_llu(T *__restrict__ A, long lda, i
 // ---------------------------------------------------------------- driver
 
 // Set by lu_factor_lowp for its duration: FP32 panels are then factored with
-// tournament pivoting (tslu.hip) in this workspace, one epoch per panel.
+// tournament pivoting (tslu.hip) in this workspace.
 static thread_local void *g_tslu_work = nullptr;
-static thread_local unsigned g_tslu_epoch = 0;
 
 template <class T>
 static int panel(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, int64_t *ipiv,
                  int64_t *info, size_t n, const LuWork &lw);
 
 template <class T>
+static T *linv_of(const LuWork &lw, size_t j0);
+
+// The panel by tournament pivoting when lu_factor_lowp asked for it (the
+// tournament also writes L11^-1: *linv_done), else by partial pivoting.
+template <class T>
 static int panel_any(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, int64_t *ipiv,
-                     int64_t *info, size_t n, const LuWork &lw) {
+                     int64_t *info, size_t n, const LuWork &lw, bool *linv_done) {
+    *linv_done = false;
     if constexpr (std::is_same<T, float>::value) {
-        if (g_tslu_work && m <= tslu_max_rows())
-            return tslu_panel(s, a, lda, k, m, w, ipiv, info, n, g_tslu_work, ++g_tslu_epoch,
-                              lw.timeout);
+        if (g_tslu_work && m <= tslu_max_rows(n)) {
+            *linv_done = true;
+            return tslu_panel(s, a, lda, k, m, w, ipiv, info, n, g_tslu_work, linv_of<float>(lw, k));
+        }
     }
     return panel<T>(s, a, lda, k, m, w, ipiv, info, n, lw);
 }
@@ -893,9 +899,10 @@ static int outer_panel(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv,
                        const LuWork &lw, size_t k0, size_t kb) {
     for (size_t j0 = k0; j0 < k0 + kb; j0 += PNB) {
         const size_t jb = k0 + kb - j0 < (size_t)PNB ? k0 + kb - j0 : PNB;
-        RLG_TRY(panel_any<T>(s, a, lda, j0, n - j0, jb, ipiv, info, n, lw));
+        bool linv_done = false;
+        RLG_TRY(panel_any<T>(s, a, lda, j0, n - j0, jb, ipiv, info, n, lw, &linv_done));
         // L^-1 of the panel's diagonal block, for its two block-row solves
-        if (j0 + jb < n) RLG_TRY(tri_inv<T>(s, a, lda, j0, jb, lw));
+        if (j0 + jb < n && !linv_done) RLG_TRY(tri_inv<T>(s, a, lda, j0, jb, lw));
         // inner swaps on the other columns of the outer block
         RLG_TRY(laswp<T>(s, a, lda, ipiv, j0, jb, k0, j0, j0 + jb, k0 + kb));
         if (j0 + jb < k0 + kb) {
@@ -984,7 +991,7 @@ template int lu_factor<float>(hipStream_t, size_t, float *, size_t, int64_t *, i
 // The trailing updates are at most n x n with depth ONB.
 size_t lu_lowp_work_bytes(size_t n) {
     return align_up(lu_work_bytes(n, 4)) + align_up(gemm_lowp_pack_bytes(n, n, ONB)) +
-           tslu_work_bytes();
+           tslu_work_bytes(n);
 }
 
 int lu_factor_lowp(hipStream_t s, int prec, int pivot, size_t n, float *a, size_t lda, int64_t *ipiv,
@@ -994,11 +1001,10 @@ int lu_factor_lowp(hipStream_t s, int prec, int pivot, size_t n, float *a, size_
     g_lowp_pack = w + align_up(lu_work_bytes(n, 4));
     g_lowp_pack_bytes = gemm_lowp_pack_bytes(n, n, ONB);
     g_tslu_work = nullptr;
-    g_tslu_epoch = 0;
     int st = RLG_OK;
     if (pivot == RLG_PIVOT_TOURNAMENT) {
         g_tslu_work = w + align_up(lu_work_bytes(n, 4)) + align_up(g_lowp_pack_bytes);
-        st = tslu_reset(s, g_tslu_work);
+        st = tslu_reset(s, g_tslu_work, n);
     }
     if (st == RLG_OK)
         st = lu_factor_impl<float>(s, n, a, lda, ipiv, info, work,

@@ -7,8 +7,22 @@
 
 #include <climits>
 #include <type_traits>
+#include <utility>
 
 namespace rlg {
+
+// Calls f(std::integral_constant<int, I>) for I = 0 .. N-1 in order, so a
+// loop body sees its index as a compile-time constant and registers indexed
+// by it stay registers.
+template <class F, int... Is>
+__device__ __forceinline__ void static_for_impl(F &f, std::integer_sequence<int, Is...>) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
 
 // Value of x in lane `l` (uniform result).
 __device__ __forceinline__ int lane_get(int x, int l) { return __builtin_amdgcn_readlane(x, l); }

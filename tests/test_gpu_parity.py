@@ -142,6 +142,43 @@ def test_matvec_vs_oracle():
     assert np.abs(y - orc.matvec(a, v)).max() <= 1e-13 * np.abs(a).sum(1).max() * np.abs(v).max()
 
 
+# ---------------------------------------------------------------- permutations
+
+@pytest.mark.parametrize("name", ["left_mul_matrix", "right_mul_matrix"])
+def test_kat_permutation_products(name):
+    c = {c["name"]: c for c in KATS["permutation"]}[name]
+    p = rl.PermutationMatrix(c["perm"])
+    m = rl.Matrix(np.array(c["a"], float))
+    out = (p * m) if name == "left_mul_matrix" else (m * p)
+    assert out.to_numpy().tolist() == c["out"]
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("rows,cols", [(1, 1), (5, 3), (300, 777), (2049, 65)])
+def test_permutation_products_vs_oracle(dt, rows, cols):
+    """Row and column permutations on the device: bit-exact (data movement)."""
+    r = np.random.default_rng(rows * 7 + cols)
+    a = r.standard_normal((rows, cols)).astype(dt)
+    pr = r.permutation(rows)
+    pc = r.permutation(cols)
+    assert np.array_equal((rl.PermutationMatrix(pr) * rl.Matrix(a)).to_numpy(), orc.permute_rows(pr, a))
+    assert np.array_equal((rl.Matrix(a) * rl.PermutationMatrix(pc)).to_numpy(), orc.permute_cols(pc, a))
+
+
+def test_permutation_product_errors():
+    a = rl.Matrix(np.ones((3, 4)))
+    with pytest.raises(rl.ShapePanic):
+        rl.PermutationMatrix([0, 1]) * a          # perm size != rows
+    with pytest.raises(rl.ShapePanic):
+        a * rl.PermutationMatrix([0, 1, 2])       # perm size != cols
+    with pytest.raises(rl.Error) as e:
+        rl.PermutationMatrix([0, 0, 1]) * a       # not a permutation
+    assert e.value.kind() == rl.ErrorKind.InvalidPermutation
+    with pytest.raises(rl.Error) as e:
+        a * rl.PermutationMatrix([0, 1, 2, 9])    # out of range
+    assert e.value.kind() == rl.ErrorKind.InvalidPermutation
+
+
 # ---------------------------------------------------------------- LU KATs
 
 @pytest.mark.parametrize("case", KATS["lu_exact"], ids=lambda c: c["name"])

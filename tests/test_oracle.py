@@ -369,6 +369,23 @@ def test_kat_permutations():
         assert orc.perm_parity(cc["perm"]) == cc["det"]
 
 
+def test_kat_permutation_products():
+    """P * M and M * P (impl_permutation_mul.rs tests), and both against the
+    full-matrix products they equal: P * M = as_matrix(P) @ M, and M * P =
+    M @ as_matrix(P)^T (the reference's M * P scatters columns to perm[j])."""
+    cases = {c["name"]: c for c in KATS["permutation"]}
+    for name, fn in (("left_mul_matrix", orc.permute_rows), ("right_mul_matrix", orc.permute_cols)):
+        c = cases[name]
+        assert fn(c["perm"], np.array(c["a"], float)).tolist() == c["out"]
+    r = np.random.default_rng(3)
+    for n, m in ((1, 4), (7, 5), (30, 11)):
+        p = r.permutation(n)
+        a = r.standard_normal((n, m))
+        assert np.array_equal(orc.permute_rows(p, a), orc.perm_matrix(p) @ a)
+        b = r.standard_normal((m, n))
+        assert np.array_equal(orc.permute_cols(p, b), b @ orc.perm_matrix(p).T)
+
+
 def test_perm_parity_brute_force():
     """Parity from permute_by_swap equals (-1)^(inversions) on random permutations."""
     r = rng(9)
