@@ -184,6 +184,61 @@ int rlg_lu_det_f64(const rlg_lu *lu, double *det);
 int rlg_lu_inverse_f64(const rlg_lu *lu, double *inv, size_t ld);
 void rlg_lu_free(rlg_lu *lu);
 
+/* ---------------------------------------------------------------- FullPivLu
+ * FullPivLu (src/matrix/decomposition/lu.rs:339-599): P A Q = L U with
+ * complete pivoting.  The packed factor, p and q are identical to the
+ * reference's (every element is computed with the reference's operations;
+ * the pivot is the first maximal |value| in row-major order). */
+typedef struct rlg_fplu rlg_fplu;
+
+/* FullPivLu::decompose (lu.rs:383-432).  rows != cols -> RLG_E_SHAPE
+ * (lu.rs:384-386).  A singular matrix is not an error: a zero pivot ends
+ * the elimination (lu.rs:405-407).  Orders above 46340 -> RLG_E_INVALID_ARG. */
+int rlg_fplu_decompose_f64(size_t rows, size_t cols, const double *a,
+                           size_t lda, rlg_fplu **out);
+int rlg_fplu_decompose_f32(size_t rows, size_t cols, const float *a,
+                           size_t lda, rlg_fplu **out);
+/* Factors the device matrix `a` in place; the handle refers to it. */
+int rlg_fplu_decompose_f64_dev(size_t n, double *a, size_t lda, void *stream,
+                               rlg_fplu **out);
+/* FullPivLu::solve (lu.rs:458-464) on nrhs vectors.  b_len != n ->
+ * RLG_E_SHAPE; a diagonal entry of U below epsilon in magnitude ->
+ * RLG_E_DIV_BY_ZERO (back_substitution, src/matrix/mod.rs:330-336).  The
+ * result is the reference's: the substitutions' result y is mapped through
+ * `&q * y` (x[q[i]] = y[i]), which is the solution of A x = b when q is its
+ * own inverse and otherwise a permutation of it. */
+int rlg_fplu_solve_f64(const rlg_fplu *lu, size_t b_len, size_t nrhs,
+                       const double *b, size_t ldb, double *x, size_t ldx);
+int rlg_fplu_solve_f32(const rlg_fplu *lu, size_t b_len, size_t nrhs,
+                       const float *b, size_t ldb, float *x, size_t ldx);
+/* As rlg_fplu_solve_f64 but mapped through q^-1: the solution of A x = b
+ * for every q (no counterpart in the reference, whose solve differs from it
+ * when q is not its own inverse). */
+int rlg_fplu_solve_true_f64(const rlg_fplu *lu, size_t b_len, size_t nrhs,
+                            const double *b, size_t ldb, double *x,
+                            size_t ldx);
+/* Device form of both: true_solution = 0 gives rlg_fplu_solve_f64's result. */
+int rlg_fplu_solve_f64_dev(const rlg_fplu *lu, size_t nrhs, const double *b,
+                           size_t ldb, double *x, size_t ldx,
+                           int true_solution, void *stream);
+int rlg_fplu_size(const rlg_fplu *lu, size_t *n);
+/* The stored p and q (lu.rs:425-429), n entries each (either may be NULL). */
+int rlg_fplu_perms(const rlg_fplu *lu, size_t *p, size_t *q);
+/* Elimination steps taken: n, or the step whose pivot was zero. */
+int rlg_fplu_steps(const rlg_fplu *lu, size_t *steps);
+/* Decomposition::unpack (lu.rs:348-361): unit-lower L and upper U, n x n
+ * row-major (ld = n); p and q from rlg_fplu_perms. */
+int rlg_fplu_unpack_f64(const rlg_fplu *lu, double *l, double *u);
+int rlg_fplu_unpack_f32(const rlg_fplu *lu, float *l, float *u);
+/* FullPivLu::rank (lu.rs:544-557), ::is_invertible (lu.rs:582-594),
+ * ::det (lu.rs:505-517), ::inverse (lu.rs:472-497: RLG_E_DIV_BY_ZERO when
+ * not invertible, else one solve per unit vector). */
+int rlg_fplu_rank(const rlg_fplu *lu, size_t *rank);
+int rlg_fplu_is_invertible(const rlg_fplu *lu, int *yes);
+int rlg_fplu_det_f64(const rlg_fplu *lu, double *det);
+int rlg_fplu_inverse_f64(const rlg_fplu *lu, double *inv, size_t ld);
+void rlg_fplu_free(rlg_fplu *lu);
+
 /* ---------------------------------------------------------------- Cholesky
  * Cholesky (src/matrix/decomposition/cholesky.rs:94-246).  Only the lower
  * triangle of the input is read (cholesky.rs:141-148). */

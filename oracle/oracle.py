@@ -48,6 +48,17 @@ def lib():
             f = getattr(_lib, f"orc_chol_inverse_{suf}"); f.restype = i32; f.argtypes = [sz, P, P]
             f = getattr(_lib, f"orc_lu_unpack_{suf}"); f.restype = None; f.argtypes = [sz, P, P, P]
             f = getattr(_lib, f"orc_chol_unpack_{suf}"); f.restype = None; f.argtypes = [sz, P]
+            f = getattr(_lib, f"orc_fplu_decompose_{suf}"); f.restype = i32
+            f.argtypes = [sz, P, C.POINTER(sz), C.POINTER(sz), C.POINTER(i64), C.POINTER(i64),
+                          C.POINTER(sz)]
+            f = getattr(_lib, f"orc_fplu_solve_{suf}"); f.restype = i32
+            f.argtypes = [sz, P, C.POINTER(sz), C.POINTER(sz), P, P]
+            f = getattr(_lib, f"orc_fplu_rank_{suf}"); f.restype = sz; f.argtypes = [sz, P]
+            f = getattr(_lib, f"orc_fplu_is_invertible_{suf}"); f.restype = i32; f.argtypes = [sz, P]
+            f = getattr(_lib, f"orc_fplu_det_{suf}"); f.restype = ct
+            f.argtypes = [sz, P, C.POINTER(sz), C.POINTER(sz)]
+            f = getattr(_lib, f"orc_fplu_inverse_{suf}"); f.restype = i32
+            f.argtypes = [sz, P, C.POINTER(sz), C.POINTER(sz), P]
         _lib.orc_perm_inverse.restype = None
         _lib.orc_perm_inverse.argtypes = [sz, C.POINTER(sz), C.POINTER(sz)]
         _lib.orc_perm_parity.restype = i32
@@ -316,3 +327,72 @@ def gemm_blocked(a, b):
     c = np.empty((m, n))
     lib().orc_gemm_blocked_f64(m, k, n, _ptr(a), k, _ptr(b), n, _ptr(c))
     return c
+
+
+def fplu_decompose(a, dtype=np.float64):
+    """FullPivLu::decompose (src/matrix/decomposition/lu.rs:383-432).
+
+    Returns (packed lu, stored p, stored q, step pivot rows, step pivot
+    columns, steps taken); the step arrays hold the first `steps` entries."""
+    lu = _f(a, dtype).copy()
+    n = lu.shape[0] if lu.ndim == 2 else 0
+    if lu.ndim != 2 or lu.shape[1] != n:
+        raise OracleError(E_SHAPE)
+    p = np.zeros(n, dtype=np.uintp)
+    q = np.zeros(n, dtype=np.uintp)
+    prow = np.zeros(max(n, 1), dtype=np.int64)
+    pcol = np.zeros(max(n, 1), dtype=np.int64)
+    steps = C.c_size_t(0)
+    st = getattr(lib(), f"orc_fplu_decompose_{_suf(dtype)}")(
+        n, _ptr(lu), _szp(p), _szp(q), prow.ctypes.data_as(C.POINTER(C.c_int64)),
+        pcol.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(steps))
+    if st != OK:
+        raise OracleError(st)
+    k = steps.value
+    return lu, p.astype(np.int64), q.astype(np.int64), prow[:k].copy(), pcol[:k].copy(), k
+
+
+def _fplu_args(lu, p, q):
+    dt = lu.dtype
+    return (_f(lu, dt), np.ascontiguousarray(p, dtype=np.uintp), np.ascontiguousarray(q, dtype=np.uintp))
+
+
+def fplu_solve(lu, p, q, b):
+    """FullPivLu::solve (src/matrix/decomposition/lu.rs:458-464)."""
+    lu, p, q = _fplu_args(lu, p, q)
+    b = _f(b, lu.dtype)
+    x = np.empty_like(b)
+    st = getattr(lib(), f"orc_fplu_solve_{_suf(lu.dtype)}")(lu.shape[0], _ptr(lu), _szp(p), _szp(q),
+                                                           _ptr(b), _ptr(x))
+    if st != OK:
+        raise OracleError(st)
+    return x
+
+
+def fplu_rank(lu):
+    """FullPivLu::rank (src/matrix/decomposition/lu.rs:544-557)."""
+    lu = _f(lu, lu.dtype)
+    return getattr(lib(), f"orc_fplu_rank_{_suf(lu.dtype)}")(lu.shape[0], _ptr(lu))
+
+
+def fplu_is_invertible(lu):
+    """FullPivLu::is_invertible (src/matrix/decomposition/lu.rs:582-594)."""
+    lu = _f(lu, lu.dtype)
+    return bool(getattr(lib(), f"orc_fplu_is_invertible_{_suf(lu.dtype)}")(lu.shape[0], _ptr(lu)))
+
+
+def fplu_det(lu, p, q):
+    """FullPivLu::det (src/matrix/decomposition/lu.rs:505-517)."""
+    lu, p, q = _fplu_args(lu, p, q)
+    return getattr(lib(), f"orc_fplu_det_{_suf(lu.dtype)}")(lu.shape[0], _ptr(lu), _szp(p), _szp(q))
+
+
+def fplu_inverse(lu, p, q):
+    """FullPivLu::inverse (src/matrix/decomposition/lu.rs:472-497)."""
+    lu, p, q = _fplu_args(lu, p, q)
+    inv = np.empty_like(lu)
+    st = getattr(lib(), f"orc_fplu_inverse_{_suf(lu.dtype)}")(lu.shape[0], _ptr(lu), _szp(p), _szp(q),
+                                                             _ptr(inv))
+    if st != OK:
+        raise OracleError(st)
+    return inv

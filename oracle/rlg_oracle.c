@@ -381,6 +381,144 @@ ORC_DEFINE(double, f64, DBL_EPSILON, sqrt)
 ORC_DEFINE(float, f32, FLT_EPSILON, sqrtf)
 
 /* ------------------------------------------------------------------------ */
+/* FullPivLu (src/matrix/decomposition/lu.rs:339-599)                        */
+/* ------------------------------------------------------------------------ */
+
+#define ORC_DEFINE_FPLU(T, SUF, EPS)                                            \
+                                                                               \
+/* FullPivLu::decompose (lu.rs:383-432) with select_pivot (lu.rs:364-380),   \
+ * gaussian_elimination (lu.rs:603-616), BaseMatrixMut::swap_rows and        \
+ * swap_cols (src/matrix/base/mod.rs:1394-1462).  select_pivot scans the     \
+ * bottom-right corner row by row with a strict `>` starting from           \
+ * (index, index), so the pivot is the first maximal |value| in row-major    \
+ * order.  A zero pivot ends the loop (lu.rs:405-407).  Outputs: `lu` the    \
+ * packed factors; p and q the stored permutations (the inverses of the     \
+ * swap-built ones, lu.rs:425-429); prow/pcol (optional) the pivot row and   \
+ * column of each step taken; *steps the number of steps taken. */           \
+int orc_fplu_decompose_##SUF(size_t n, T *lu, size_t *p, size_t *q,            \
+                             int64_t *prow, int64_t *pcol, size_t *steps)      \
+{                                                                              \
+    size_t *rp = (size_t *)malloc((n ? n : 1) * sizeof(size_t));               \
+    size_t *rq = (size_t *)malloc((n ? n : 1) * sizeof(size_t));               \
+    for (size_t i = 0; i < n; ++i) { rp[i] = i; rq[i] = i; }                   \
+    size_t index = 0;                                                          \
+    for (; index < n; ++index) {                                               \
+        size_t piv_row = index, piv_col = index;                               \
+        T piv_val = lu[index * n + index];                                     \
+        for (size_t row = index; row < n; ++row)                               \
+            for (size_t col = index; col < n; ++col) {                         \
+                T val = lu[row * n + col];                                     \
+                if ((T)fabs((double)val) > (T)fabs((double)piv_val)) {         \
+                    piv_val = val; piv_row = row; piv_col = col;               \
+                }                                                              \
+            }                                                                  \
+        if ((T)fabs((double)piv_val) == (T)0) break;                           \
+        if (prow) prow[index] = (int64_t)piv_row;                              \
+        if (pcol) pcol[index] = (int64_t)piv_col;                              \
+        if (index != piv_row)                                                  \
+            for (size_t j = 0; j < n; ++j) {                                   \
+                T t = lu[index * n + j];                                       \
+                lu[index * n + j] = lu[piv_row * n + j];                       \
+                lu[piv_row * n + j] = t;                                       \
+            }                                                                  \
+        if (index != piv_col)                                                  \
+            for (size_t i = 0; i < n; ++i) {                                   \
+                T t = lu[i * n + index];                                       \
+                lu[i * n + index] = lu[i * n + piv_col];                       \
+                lu[i * n + piv_col] = t;                                       \
+            }                                                                  \
+        { size_t t = rp[index]; rp[index] = rp[piv_row]; rp[piv_row] = t; }    \
+        { size_t t = rq[index]; rq[index] = rq[piv_col]; rq[piv_col] = t; }    \
+        T pv = lu[index * n + index];                                          \
+        for (size_t i = index + 1; i < n; ++i) {                               \
+            T mult = lu[i * n + index] / pv;                                   \
+            lu[i * n + index] = mult;                                          \
+            for (size_t j = index + 1; j < n; ++j)                             \
+                lu[i * n + j] = lu[i * n + j] - mult * lu[index * n + j];      \
+        }                                                                      \
+    }                                                                          \
+    if (steps) *steps = index;                                                 \
+    orc_perm_inverse(n, rp, p);                                                \
+    orc_perm_inverse(n, rq, q);                                                \
+    free(rp); free(rq);                                                        \
+    return ORC_OK;                                                             \
+}                                                                              \
+                                                                               \
+/* FullPivLu::solve (lu.rs:458-464): forward substitution on &p * b, back   \
+ * substitution, then &q * x (y[q[i]] = x[i]). */                            \
+int orc_fplu_solve_##SUF(size_t n, const T *lu, const size_t *p,               \
+                         const size_t *q, const T *b, T *x)                    \
+{                                                                              \
+    T *y = (T *)malloc((n ? n : 1) * sizeof(T));                               \
+    for (size_t i = 0; i < n; ++i) y[p[i]] = b[i];                             \
+    orc_lu_forward_substitution_##SUF(n, lu, y);                               \
+    int st = orc_back_substitution_##SUF(n, lu, n, y);                         \
+    if (st == ORC_OK)                                                          \
+        for (size_t i = 0; i < n; ++i) x[q[i]] = y[i];                         \
+    free(y);                                                                   \
+    return st;                                                                 \
+}                                                                              \
+                                                                               \
+/* FullPivLu::epsilon (lu.rs:596-598): |lu[0][0]| * eps, or eps for n = 0. */  \
+static T orc_fplu_eps_##SUF(size_t n, const T *lu)                             \
+{                                                                              \
+    return (n ? (T)fabs((double)lu[0]) : (T)1) * EPS;                          \
+}                                                                              \
+                                                                               \
+/* FullPivLu::rank (lu.rs:544-557): leading diagonal entries above eps. */   \
+size_t orc_fplu_rank_##SUF(size_t n, const T *lu)                              \
+{                                                                              \
+    T eps = orc_fplu_eps_##SUF(n, lu);                                         \
+    size_t rank = 0;                                                           \
+    for (size_t i = 0; i < n; ++i) {                                           \
+        if ((T)fabs((double)lu[i * n + i]) > eps) rank = rank + 1;             \
+        else break;                                                            \
+    }                                                                          \
+    return rank;                                                               \
+}                                                                              \
+                                                                               \
+/* FullPivLu::is_invertible (lu.rs:582-594): the last diagonal entry above   \
+ * eps; true for n = 0. */                                                   \
+int orc_fplu_is_invertible_##SUF(size_t n, const T *lu)                        \
+{                                                                              \
+    if (n == 0) return 1;                                                      \
+    return (T)fabs((double)lu[(n - 1) * n + (n - 1)]) > orc_fplu_eps_##SUF(n, lu); \
+}                                                                              \
+                                                                               \
+/* FullPivLu::det (lu.rs:505-517): diagonal product folded from one, times  \
+ * the parities of p and q. */                                               \
+T orc_fplu_det_##SUF(size_t n, const T *lu, const size_t *p, const size_t *q)  \
+{                                                                              \
+    T u_det = 1;                                                               \
+    for (size_t i = 0; i < n; ++i) u_det = u_det * lu[i * n + i];              \
+    T p_det = orc_perm_parity(n, p) > 0 ? (T)1 : (T)0 - (T)1;                  \
+    T q_det = orc_perm_parity(n, q) > 0 ? (T)1 : (T)0 - (T)1;                  \
+    return p_det * u_det * q_det;                                              \
+}                                                                              \
+                                                                               \
+/* FullPivLu::inverse (lu.rs:472-497): DivByZero when not invertible, else   \
+ * one solve per unit vector, stored as column i. */                         \
+int orc_fplu_inverse_##SUF(size_t n, const T *lu, const size_t *p,             \
+                           const size_t *q, T *inv)                            \
+{                                                                              \
+    if (!orc_fplu_is_invertible_##SUF(n, lu)) return ORC_E_DIV_BY_ZERO;        \
+    T *e = (T *)calloc(n ? n : 1, sizeof(T));                                  \
+    T *col = (T *)malloc((n ? n : 1) * sizeof(T));                             \
+    int st = ORC_OK;                                                           \
+    for (size_t i = 0; i < n && st == ORC_OK; ++i) {                           \
+        e[i] = 1;                                                              \
+        st = orc_fplu_solve_##SUF(n, lu, p, q, e, col);                        \
+        for (size_t j = 0; j < n; ++j) inv[j * n + i] = col[j];                \
+        e[i] = 0;                                                              \
+    }                                                                          \
+    free(e); free(col);                                                        \
+    return st;                                                                 \
+}
+
+ORC_DEFINE_FPLU(double, f64, DBL_EPSILON)
+ORC_DEFINE_FPLU(float, f32, FLT_EPSILON)
+
+/* ------------------------------------------------------------------------ */
 /* Baseline GEMM stand-in                                                    */
 /* ------------------------------------------------------------------------ */
 

@@ -11,6 +11,8 @@ reference's error behaviour:
   &m * &v  (src/matrix/impl_ops.rs:298-315)         m * v with a Vector
   PartialPivLu::decompose / solve / unpack / det   PartialPivLu.decompose(m).solve(b) ...
   / inverse (src/matrix/decomposition/lu.rs)
+  FullPivLu::decompose / solve / unpack / det /    FullPivLu.decompose(m).solve(b) ...
+  inverse / rank / is_invertible (lu.rs:339-599)
   Cholesky::decompose / solve / unpack / det /     Cholesky.decompose(m).solve(b) ...
   inverse (src/matrix/decomposition/cholesky.rs)
   Matrix::solve / inverse / det                    Matrix.solve / inverse / det
@@ -27,6 +29,7 @@ import numpy as np
 from . import _lib
 
 __all__ = ["Matrix", "MatrixSlice", "Vector", "PermutationMatrix", "PartialPivLu", "LUP",
+           "FullPivLu", "LUPQ",
            "Cholesky", "Error", "ErrorKind", "ShapePanic", "solve_mixed", "MixedOptions"]
 
 
@@ -410,6 +413,106 @@ class PartialPivLu:
     def inverse(self):
         inv = np.empty((self._n, self._n))
         _check(_lib.load().rlg_lu_inverse_f64(self._h, inv.ctypes.data, max(self._n, 1)))
+        return Matrix(inv)
+
+
+LUPQ = namedtuple("LUPQ", ["l", "u", "p", "q"])
+
+
+class FullPivLu:
+    """FullPivLu<T> (src/matrix/decomposition/lu.rs:339-599), factor on the GPU.
+
+    solve() returns the reference's result (lu.rs:458-464), which maps the
+    substitutions' result through `&q * y`: that is the solution of A x = b
+    when q is its own inverse and a permutation of it otherwise.
+    solve_true() maps through q^-1 and returns the solution for every q."""
+
+    def __init__(self, handle, n, dtype):
+        self._h = handle
+        self._n = n
+        self.dtype = dtype
+
+    @staticmethod
+    def decompose(matrix):
+        m = matrix if isinstance(matrix, Matrix) else Matrix(matrix)
+        h = C.c_void_p()
+        lib = _lib.load()
+        fn = lib.rlg_fplu_decompose_f32 if m.dtype == np.float32 else lib.rlg_fplu_decompose_f64
+        _check(fn(m.rows(), m.cols(), m._base_ptr(), m.row_stride(), C.byref(h)))
+        return FullPivLu(h, m.rows(), m.dtype)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.load().rlg_fplu_free(h)
+            self._h = None
+
+    def _solve(self, b, true_solution):
+        bm = np.ascontiguousarray(b, dtype=self.dtype)
+        one = bm.ndim == 1
+        bm = bm.reshape(1, -1) if one else bm
+        x = np.empty_like(bm)
+        lib = _lib.load()
+        if true_solution:
+            fn = lib.rlg_fplu_solve_true_f64
+        else:
+            fn = lib.rlg_fplu_solve_f32 if self.dtype == np.float32 else lib.rlg_fplu_solve_f64
+        ld = max(bm.shape[1], 1)
+        _check(fn(self._h, bm.shape[1], bm.shape[0], bm.ctypes.data, ld, x.ctypes.data, ld))
+        return x[0] if one else x
+
+    def solve(self, b):
+        bv = b if isinstance(b, Vector) else Vector(b, dtype=self.dtype)
+        return Vector(self._solve(bv.data(), False))
+
+    def solve_many(self, b):
+        """nrhs right-hand sides at once: b is (nrhs, n), one vector per row."""
+        return self._solve(b, False)
+
+    def solve_true(self, b):
+        bv = b if isinstance(b, Vector) else Vector(b)
+        return Vector(self._solve(bv.data(), True))
+
+    def perms(self):
+        p = np.empty(self._n, dtype=np.uintp)
+        q = np.empty(self._n, dtype=np.uintp)
+        _check(_lib.load().rlg_fplu_perms(self._h, p.ctypes.data, q.ctypes.data))
+        return p.astype(np.int64), q.astype(np.int64)
+
+    def steps(self):
+        s = C.c_size_t()
+        _check(_lib.load().rlg_fplu_steps(self._h, C.byref(s)))
+        return s.value
+
+    def unpack(self):
+        """Decomposition::unpack (lu.rs:348-361)."""
+        n = self._n
+        l = np.empty((n, n), dtype=self.dtype)
+        u = np.empty((n, n), dtype=self.dtype)
+        lib = _lib.load()
+        fn = lib.rlg_fplu_unpack_f32 if self.dtype == np.float32 else lib.rlg_fplu_unpack_f64
+        _check(fn(self._h, l.ctypes.data, u.ctypes.data))
+        p, q = self.perms()
+        return LUPQ(Matrix(l), Matrix(u), PermutationMatrix(p), PermutationMatrix(q))
+
+    def rank(self):
+        r = C.c_size_t()
+        _check(_lib.load().rlg_fplu_rank(self._h, C.byref(r)))
+        return r.value
+
+    def is_invertible(self):
+        y = C.c_int()
+        _check(_lib.load().rlg_fplu_is_invertible(self._h, C.byref(y)))
+        return bool(y.value)
+
+    def det(self):
+        d = C.c_double()
+        _check(_lib.load().rlg_fplu_det_f64(self._h, C.byref(d)))
+        return d.value
+
+    def inverse(self):
+        inv = np.empty((self._n, self._n))
+        _check(_lib.load().rlg_fplu_inverse_f64(self._h, inv.ctypes.data, max(self._n, 1)))
         return Matrix(inv)
 
 

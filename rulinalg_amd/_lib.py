@@ -105,6 +105,23 @@ SIGNATURES = {
     "rlg_permute_rows_f64_dev": (i32, [sz, vp, sz, vp, sz, vp, sz, vp]),
     "rlg_permute_cols_f64_dev": (i32, [sz, sz, vp, vp, sz, vp, sz, vp]),
     "rlg_lu_free": (None, [vp]),
+    "rlg_fplu_decompose_f64": (i32, [sz, sz, vp, sz, C.POINTER(vp)]),
+    "rlg_fplu_decompose_f32": (i32, [sz, sz, vp, sz, C.POINTER(vp)]),
+    "rlg_fplu_decompose_f64_dev": (i32, [sz, vp, sz, vp, C.POINTER(vp)]),
+    "rlg_fplu_solve_f64": (i32, [vp, sz, sz, vp, sz, vp, sz]),
+    "rlg_fplu_solve_f32": (i32, [vp, sz, sz, vp, sz, vp, sz]),
+    "rlg_fplu_solve_true_f64": (i32, [vp, sz, sz, vp, sz, vp, sz]),
+    "rlg_fplu_solve_f64_dev": (i32, [vp, sz, vp, sz, vp, sz, i32, vp]),
+    "rlg_fplu_size": (i32, [vp, PSZ]),
+    "rlg_fplu_perms": (i32, [vp, vp, vp]),
+    "rlg_fplu_steps": (i32, [vp, PSZ]),
+    "rlg_fplu_unpack_f64": (i32, [vp, vp, vp]),
+    "rlg_fplu_unpack_f32": (i32, [vp, vp, vp]),
+    "rlg_fplu_rank": (i32, [vp, PSZ]),
+    "rlg_fplu_is_invertible": (i32, [vp, C.POINTER(i32)]),
+    "rlg_fplu_det_f64": (i32, [vp, PD]),
+    "rlg_fplu_inverse_f64": (i32, [vp, vp, sz]),
+    "rlg_fplu_free": (None, [vp]),
     "rlg_chol_decompose_f64": (i32, [sz, sz, vp, sz, C.POINTER(vp)]),
     "rlg_chol_decompose_f32": (i32, [sz, sz, vp, sz, C.POINTER(vp)]),
     "rlg_chol_decompose_f64_dev": (i32, [sz, vp, sz, vp, C.POINTER(vp)]),

@@ -105,6 +105,21 @@ int lu_solve(hipStream_t s, size_t n, const T *lu, size_t lda,
              const int64_t *raw, size_t nrhs, const T *b, size_t ldb, T *x,
              size_t ldx, T *tmp);
 
+// ------------------------------------------------------------------ FullPivLu
+// LU with complete pivoting of the n x n matrix `a` in place (fplu.hip), in
+// a workspace of fplu_work_bytes(n, elem) bytes; fplu_result then copies the
+// step pivots (row and column of each step) and the number of steps taken
+// to the host.
+size_t fplu_work_bytes(size_t n, size_t elem);
+template <class T>
+int fplu_factor(hipStream_t s, size_t n, T *a, size_t lda, void *work);
+int fplu_result(hipStream_t s, size_t n, void *work, size_t elem, int64_t *prow,
+                int64_t *pcol, int64_t *steps);
+// x[r][perm[i]] = y[r][i] for nrhs vectors of length n.
+template <class T>
+int scatter_vec(hipStream_t s, size_t n, size_t nrhs, const int64_t *perm, const T *y,
+                size_t ldy, T *x, size_t ldx);
+
 // ------------------------------------------------------------------ Cholesky
 size_t chol_work_bytes(size_t n, size_t elem);
 template <class T>

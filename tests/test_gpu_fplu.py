@@ -1,0 +1,165 @@
+"""GPU: FullPivLu (src/matrix/decomposition/lu.rs:339-599) through librlgpu's
+C ABI against the CPU oracle and the reference's own tests.
+
+Tolerances (stated here, used below):
+  * bit-exact: the packed factor, p, q and the number of elimination steps at
+    every order (every element is computed with the reference's operations,
+    and complete pivoting leaves no blocking to change their order), also on
+    matrices full of exact ties;
+  * the reference's KATs under the reference's comparators;
+  * solves: the substitutions run on the GPU in their own order, so the
+    result is compared with the oracle's as ||x - x_oracle||inf /
+    ||x_oracle||inf <= max(1e-13, 16 n kappa_inf u), and solve_true has
+    normwise backward error <= 1e-13.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import rulinalg_amd as rl
+from oracle import oracle as orc
+from rulinalg_amd import _lib
+from tests.refcmp import assert_eq
+
+pytestmark = pytest.mark.gpu
+
+KATS = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "kats.json")))
+U = np.finfo(np.float64).eps
+
+
+@pytest.fixture(scope="module", autouse=True)
+def lib():
+    lib = _lib.load()
+    assert lib.rlg_set_device(0) == 0, _lib.last_error()
+    return lib
+
+
+def rng(seed):
+    return np.random.Generator(np.random.Philox(seed))
+
+
+def fwd_tol(a):
+    n = a.shape[0]
+    return max(1e-13, 16 * n * np.linalg.cond(a, np.inf) * U)
+
+
+def packed(f):
+    l, u, _, _ = f.unpack()
+    return np.tril(l.to_numpy(), -1) + u.to_numpy()
+
+
+def check_identical(a, dtype=np.float64):
+    f = rl.FullPivLu.decompose(rl.Matrix(np.asarray(a, dtype=dtype)))
+    lu_or, p_or, q_or, prow, pcol, steps = orc.fplu_decompose(a, dtype=dtype)
+    p, q = f.perms()
+    assert f.steps() == steps
+    assert np.array_equal(p, p_or) and np.array_equal(q, q_or)
+    assert np.array_equal(packed(f), lu_or), "packed factor differs from the oracle"
+    return f, lu_or, p_or, q_or
+
+
+@pytest.mark.parametrize("case", KATS["fplu"], ids=lambda c: c["name"])
+def test_kat_fplu(case):
+    a = np.array(case["a"], float).reshape(len(case["a"]), -1) if case["a"] else np.zeros((0, 0))
+    f = rl.FullPivLu.decompose(rl.Matrix(a))
+    if case.get("reconstruct"):
+        l, u, p, q = f.unpack()
+        # p.inverse() * l * u * q.inverse() with the reference's permutation products
+        rec = (l.to_numpy() @ u.to_numpy())
+        rec = orc.permute_cols(orc.perm_inverse(q.into_vec()), orc.permute_rows(orc.perm_inverse(p.into_vec()), rec))
+        assert_eq(rec, a, case["comp"], what=case["name"])
+    if "rank" in case:
+        assert f.rank() == case["rank"]
+    if "invertible" in case:
+        assert f.is_invertible() == case["invertible"]
+    if "det" in case:
+        assert f.det() == case["det"]
+    if "x" in case:
+        assert_eq(f.solve(rl.Vector(case["b"])).data(), case["x"], case["comp"], tol=case.get("tol"),
+                  what=case["name"])
+    if "inv" in case:
+        assert_eq(f.inverse().to_numpy(), case["inv"], case["comp"], what=case["name"])
+    if case.get("inverse_err") == "DivByZero":
+        with pytest.raises(rl.Error) as e:
+            f.inverse()
+        assert e.value.kind() == rl.ErrorKind.DivByZero
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 17, 64, 65, 100, 257, 600, 1024])
+def test_fplu_bit_identical_to_oracle(n):
+    a = rng(100 + n).standard_normal((n, n))
+    check_identical(a)
+
+
+def test_fplu_f32_bit_identical():
+    a = rng(7).standard_normal((150, 150)).astype(np.float32)
+    check_identical(a, np.float32)
+
+
+@pytest.mark.parametrize("n", [9, 70, 300])
+def test_fplu_exact_ties(n):
+    """Small integers: many exact ties in |value| at every step, so the pivot
+    depends on the first-in-row-major-order rule across tiles and
+    workgroups."""
+    a = rng(200 + n).integers(-3, 4, (n, n)).astype(np.float64)
+    check_identical(a)
+
+
+@pytest.mark.parametrize("rank", [0, 1, 5])
+def test_fplu_singular_stops_like_the_reference(rank):
+    """A rank-r block scattered through an order-90 matrix: the trailing block
+    is exactly zero after r steps, the elimination ends there, solve fails
+    with DivByZero and inverse reports non-invertible, as the oracle does."""
+    r = rng(300 + rank)
+    a = np.zeros((90, 90))
+    if rank:
+        a[:rank, :rank] = r.standard_normal((rank, rank))
+    a = a[r.permutation(90)][:, r.permutation(90)]
+    f, lu_or, p_or, q_or = check_identical(a)
+    assert f.steps() == rank and f.rank() == orc.fplu_rank(lu_or) == rank
+    assert not f.is_invertible()
+    with pytest.raises(rl.Error) as e:
+        f.solve(rl.Vector(np.ones(90)))
+    assert e.value.kind() == rl.ErrorKind.DivByZero
+    with pytest.raises(rl.Error) as e:
+        f.inverse()
+    assert e.value.kind() == rl.ErrorKind.DivByZero
+
+
+@pytest.mark.parametrize("n", [4, 33, 200])
+def test_fplu_solve_det_inverse_vs_oracle(n):
+    a = rng(400 + n).standard_normal((n, n))
+    f, lu_or, p_or, q_or = check_identical(a)
+    b = rng(500 + n).standard_normal(n)
+    x = f.solve(rl.Vector(b)).data()
+    x_or = orc.fplu_solve(lu_or, p_or, q_or, b)
+    assert np.abs(x - x_or).max() / np.abs(x_or).max() <= fwd_tol(a)
+    xt = f.solve_true(rl.Vector(b)).data()
+    assert np.abs(b - a @ xt).max() / (np.abs(a).sum(1).max() * np.abs(xt).max() + np.abs(b).max()) <= 1e-13
+    # the reference's solve is a permutation of the true one: x[q[q[j]]] = xt[j]
+    assert np.abs(x[q_or[q_or]] - xt).max() / np.abs(xt).max() <= fwd_tol(a)
+    d, d_or = f.det(), orc.fplu_det(lu_or, p_or, q_or)
+    assert d == d_or
+    inv, inv_or = f.inverse().to_numpy(), orc.fplu_inverse(lu_or, p_or, q_or)
+    assert np.abs(inv - inv_or).max() / np.abs(inv_or).max() <= fwd_tol(a)
+    assert f.rank() == orc.fplu_rank(lu_or) and f.is_invertible() == orc.fplu_is_invertible(lu_or)
+
+
+def test_fplu_multi_rhs_matches_single():
+    n = 120
+    a = rng(600).standard_normal((n, n))
+    f = rl.FullPivLu.decompose(rl.Matrix(a))
+    b = rng(601).standard_normal((5, n))
+    xs = f.solve_many(b)
+    for i in range(5):
+        assert np.array_equal(xs[i], f.solve(rl.Vector(b[i])).data())
+
+
+def test_fplu_nonsquare_panics():
+    with pytest.raises(rl.ShapePanic):
+        rl.FullPivLu.decompose(rl.Matrix(np.ones((4, 3))))
+    f = rl.FullPivLu.decompose(rl.Matrix(np.eye(3)))
+    with pytest.raises(rl.ShapePanic):
+        f.solve(rl.Vector(np.ones(4)))

@@ -411,3 +411,144 @@ def test_gemm_blocked_standin():
     b = r.uniform(-1, 1, (300, 530))
     c = orc.gemm_blocked(a, b)
     assert np.abs(c - a @ b).max() < 1e-12
+
+
+# ---------------------------------------------------------------- FullPivLu
+
+def _fplu_reconstruct(lu, p, q):
+    """p.inverse() * L * U * q.inverse() from the packed factor and the stored
+    permutations with the reference's own permutation products
+    (impl_permutation_mul.rs), as its reconstruction test does (lu.rs:903-907)."""
+    l, u = orc.lu_unpack(lu)
+    return orc.permute_cols(orc.perm_inverse(q), orc.permute_rows(orc.perm_inverse(p), l @ u))
+
+
+@pytest.mark.parametrize("case", KATS["fplu"], ids=lambda c: c["name"])
+def test_kat_fplu(case):
+    a = np.array(case["a"], float).reshape(len(case["a"]), -1) if case["a"] else np.zeros((0, 0))
+    lu, p, q, prow, pcol, steps = orc.fplu_decompose(a)
+    if case.get("reconstruct"):
+        assert_eq(_fplu_reconstruct(lu, p, q), a, case["comp"], what=case["name"])
+        l, u = orc.lu_unpack(lu)
+        assert np.array_equal(np.tril(l), l) and np.array_equal(np.triu(u), u)
+    if "rank" in case:
+        assert orc.fplu_rank(lu) == case["rank"]
+    if "invertible" in case:
+        assert orc.fplu_is_invertible(lu) == case["invertible"]
+    if "det" in case:
+        assert orc.fplu_det(lu, p, q) == case["det"]
+    if "x" in case:
+        x = orc.fplu_solve(lu, p, q, np.array(case["b"], float))
+        assert_eq(x, case["x"], case["comp"], tol=case.get("tol"), what=case["name"])
+    if "inv" in case:
+        assert_eq(orc.fplu_inverse(lu, p, q), case["inv"], case["comp"], what=case["name"])
+    if case.get("inverse_err") == "DivByZero":
+        with pytest.raises(orc.OracleError) as e:
+            orc.fplu_inverse(lu, p, q)
+        assert e.value.code == orc.E_DIV_BY_ZERO
+    if a.size == 0:
+        assert orc.fplu_inverse(lu, p, q).shape == (0, 0)
+
+
+@pytest.mark.parametrize("n,seed", [(5, 1), (17, 2), (64, 3), (150, 4)])
+def test_fplu_matches_lapack_getc2(n, seed):
+    """LAPACK dgetc2 is LU with complete pivoting over the same row-major scan;
+    on Gaussian matrices (no ties, no tiny pivots) its row and column
+    interchanges must equal the oracle's step pivots and its factor the
+    oracle's packed factor to rounding."""
+    lap = pytest.importorskip("scipy.linalg.lapack")
+    a = rng(seed).standard_normal((n, n))
+    lu, p, q, prow, pcol, steps = orc.fplu_decompose(a)
+    lu_l, ipiv, jpiv, info = lap.dgetc2(a)
+    assert info == 0 and steps == n
+    assert np.array_equal(prow, ipiv) and np.array_equal(pcol, jpiv)  # scipy returns 0-based
+    assert np.abs(lu - lu_l).max() / np.abs(lu_l).max() < 1e-12 * n
+
+
+def test_fplu_complete_pivoting_bounds():
+    """Properties only complete pivoting has: every multiplier |l_ij| <= 1 and
+    every U row is dominated by its diagonal (|u_kj| <= |u_kk|), on a matrix
+    where partial (column-only) pivoting violates the second."""
+    a = np.array([[1.0, 100.0, 0.0], [2.0, 1.0, 3.0], [0.5, 4.0, 1.0]])
+    lu, p, q, prow, pcol, steps = orc.fplu_decompose(a)
+    l, u = orc.lu_unpack(lu)
+    assert np.abs(np.tril(l, -1)).max() <= 1.0
+    assert np.all(np.abs(u) <= np.abs(np.diag(u))[:, None] + 0.0)
+    lu_pp, _, _ = orc.lu_decompose(a)
+    u_pp = np.triu(lu_pp)
+    assert not np.all(np.abs(u_pp) <= np.abs(np.diag(u_pp))[:, None])
+    r = rng(13)
+    for n in (6, 40):
+        b = r.standard_normal((n, n))
+        lu, p, q, *_ = orc.fplu_decompose(b)
+        l, u = orc.lu_unpack(lu)
+        assert np.abs(np.tril(l, -1)).max() <= 1.0
+        assert np.all(np.abs(u) <= np.abs(np.diag(u))[:, None])
+        assert np.abs(_fplu_reconstruct(lu, p, q) - b).max() < 1e-13 * n
+
+
+def test_fplu_tie_goes_to_first_in_row_major_order():
+    """Equal maxima at (0, 1) and (1, 0): the strict `>` row-by-row scan keeps
+    (0, 1), so the first step swaps columns and not rows (a column-major scan
+    would swap rows instead)."""
+    a = np.array([[1.0, 5.0], [5.0, 2.0]])
+    lu, p, q, prow, pcol, steps = orc.fplu_decompose(a)
+    assert (prow[0], pcol[0]) == (0, 1)
+    assert list(q) == [1, 0] and list(p) == [0, 1]
+    # the diagonal (index, index) element is kept on a tie with it
+    b = np.array([[-5.0, 5.0], [5.0, 1.0]])
+    _, _, _, prow, pcol, _ = orc.fplu_decompose(b)
+    assert (prow[0], pcol[0]) == (0, 0)
+
+
+def test_fplu_zero_pivot_stops_elimination():
+    """A zero trailing block ends the loop (lu.rs:405-407): nothing after it is
+    swapped or eliminated, and rank counts the steps taken."""
+    a = np.zeros((4, 4))
+    a[:2, :2] = [[2.0, 1.0], [1.0, 3.0]]
+    a[3, 0] = 0.0
+    lu, p, q, prow, pcol, steps = orc.fplu_decompose(a)
+    assert steps == 2 and orc.fplu_rank(lu) == 2 and not orc.fplu_is_invertible(lu)
+    # rank 3 in an order-30 matrix with its nonzero block scattered by random
+    # row and column permutations: the trailing block is exactly zero after 3 steps
+    r = rng(14)
+    b = np.zeros((30, 30))
+    b[:3, :3] = r.standard_normal((3, 3))
+    b = b[r.permutation(30)][:, r.permutation(30)]
+    lu, p, q, prow, pcol, steps = orc.fplu_decompose(b)
+    assert steps == 3 and orc.fplu_rank(lu) == 3
+
+
+def test_fplu_det_and_solve_identities():
+    """det = det(A).  solve follows lu.rs:458-464 literally: it maps the
+    substitutions' result y through `&q * y` (x[q[i]] = y[i]) although
+    P A Q = L U makes the solution x[q^-1[i]] = y[i].  The two agree when q is
+    its own inverse (the reference's 4 x 4 tests); in general the returned x
+    satisfies A x' = b for x'[j] = x[q[q[j]]].  A restatement that applied
+    q^-1, or no column permutation, fails this test."""
+    r = rng(15)
+    for n in (1, 3, 12, 40):
+        a = r.standard_normal((n, n))
+        lu, p, q, *_ = orc.fplu_decompose(a)
+        assert abs(orc.fplu_det(lu, p, q) - np.linalg.det(a)) <= 1e-12 * abs(np.linalg.det(a)) * n
+        b = r.standard_normal(n)
+        x = orc.fplu_solve(lu, p, q, b)
+        xs = x[q[q]]
+        assert np.abs(a @ xs - b).max() < 1e-12 * n * np.abs(b).max()
+        if n >= 12:
+            assert not np.array_equal(q[q], np.arange(n))  # the case above is not vacuous
+        inv = orc.fplu_inverse(lu, p, q)
+        assert np.abs(a @ inv[q[q]] - np.eye(n)).max() < 1e-11 * n
+    # an involutive q: solve and inverse are the true ones
+    a = np.array([[1.0, 5.0, 0.5], [0.25, 2.0, 1.0], [3.0, 0.5, 1.0]])
+    lu, p, q, *_ = orc.fplu_decompose(a)
+    assert np.array_equal(q[q], np.arange(3))
+    assert np.abs(orc.fplu_inverse(lu, p, q) @ a - np.eye(3)).max() < 1e-15 * 8
+
+
+def test_fplu_f32():
+    a = rng(16).standard_normal((20, 20)).astype(np.float32)
+    lu, p, q, prow, pcol, steps = orc.fplu_decompose(a, dtype=np.float32)
+    assert lu.dtype == np.float32 and steps == 20
+    lu64, _, _, prow64, pcol64, _ = orc.fplu_decompose(a.astype(np.float64))
+    assert np.array_equal(prow, prow64) and np.array_equal(pcol, pcol64)
