@@ -33,6 +33,7 @@ METRIC = "solves/s + p99 latency at FP64 backward err ≤1e-13: mixed-refined vs
 PEAKS = {"f64": 78.6, "f32": 157.3, "f16": 2500.0, "bf16": 2500.0}  # dense MFMA TFLOP/s
 HBM_PEAK = 8000.0  # GB/s
 PREC = {"f32": 1, "f16": 2, "bf16": 3}
+PIVOT = {"partial": 0, "tournament": 1}
 
 
 def parse():
@@ -44,6 +45,8 @@ def parse():
     p.add_argument("--nrhs", type=int, default=1)
     p.add_argument("--kappa", type=float, default=1e4)
     p.add_argument("--prec", default="f32", choices=sorted(PREC))
+    p.add_argument("--pivot", default="partial", choices=sorted(PIVOT),
+                   help="pivoting of the low-precision factorization")
     p.add_argument("--plain-steps", type=int, default=None)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extras", action="store_true")
@@ -151,7 +154,7 @@ def main():
     torch.cuda.synchronize()
     log(f"inputs ready in {time.time() - t0:.1f}s")
 
-    opts = _lib.MixedOpts(PREC[args.prec], 0, 1e-13, 30, 1)
+    opts = _lib.MixedOpts(PREC[args.prec], 0, 1e-13, 30, 1, PIVOT[args.pivot])
     rep = _lib.MixedReport()
     eta = C.c_double()
 
@@ -270,7 +273,7 @@ def main():
         "dtype": (f"{args.prec} factor + f32 panels" if args.prec != "f32" else "f32 factor") + ", f64 refinement", "data": "synthetic",
         "config": {"workload": f"refined mixed-precision LU solve, A = U diag(s) V^T, n={n}, "
                                f"kappa={args.kappa:g}, {nrhs} RHS", "n": n, "nrhs": nrhs,
-                   "kappa": args.kappa, "factor_prec": args.prec, "tol": 1e-13},
+                   "kappa": args.kappa, "factor_prec": args.prec, "pivot": args.pivot, "tol": 1e-13},
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "backward_error_max": eta_max, "ir_iterations": iters, "fallbacks": fell_back,
@@ -287,7 +290,7 @@ def main():
         variants = {}
 
         def mixed_once(a, b, x, nr, prec):
-            o = _lib.MixedOpts(PREC[prec], 0, 1e-13, 30, 1)
+            o = _lib.MixedOpts(PREC[prec], 0, 1e-13, 30, 1, PIVOT[args.pivot])
             r = _lib.MixedReport()
             assert lib.rlg_synchronize() == 0
             t0 = time.perf_counter()
