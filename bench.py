@@ -30,9 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "solves/s + p99 latency at FP64 backward err ≤1e-13: mixed-refined vs plain LU"
-PEAKS = {"f64": 78.6, "f32": 157.3, "f16": 2500.0, "bf16": 2500.0}  # dense MFMA TFLOP/s
+PEAKS = {"f64": 78.6, "f32": 157.3, "f16": 2500.0, "bf16": 2500.0,
+         # three FP16 products per product of the split operands
+         "f16x3": 2500.0 / 3}  # dense MFMA TFLOP/s
 HBM_PEAK = 8000.0  # GB/s
-PREC = {"f32": 1, "f16": 2, "bf16": 3}
+PREC = {"f32": 1, "f16": 2, "bf16": 3, "f16x3": 4}
 PIVOT = {"partial": 0, "tournament": 1}
 
 
@@ -234,7 +236,8 @@ def main():
 
     # ------------------------------------------------------------ roofline
     # dominant kernel class of the mixed path in the timed region
-    gemm_classes = {"gemm_f64": "f64", "gemm_f32": "f32", "gemm_f16": "f16", "gemm_bf16": "bf16"}
+    gemm_classes = {"gemm_f64": "f64", "gemm_f32": "f32", "gemm_f16": "f16", "gemm_bf16": "bf16",
+                    "gemm_f16x3": "f16x3"}
     dom = max(prof.items(), key=lambda kv: kv[1][1])
     name, (cnt, ms, wk) = dom
     def roof_of(name, cnt, ms, wk):

@@ -280,7 +280,11 @@ enum {
     RLG_PREC_F64 = 0,   /* plain FP64: no refinement */
     RLG_PREC_F32 = 1,   /* FP32 factor, FP32 MFMA update */
     RLG_PREC_F16 = 2,   /* FP32 panels, FP16 MFMA update, FP32 accumulate */
-    RLG_PREC_BF16 = 3   /* FP32 panels, BF16 MFMA update, FP32 accumulate */
+    RLG_PREC_BF16 = 3,  /* FP32 panels, BF16 MFMA update, FP32 accumulate */
+    RLG_PREC_F16X3 = 4  /* FP32 panels; update operands split as hi + lo FP16
+                           with power-of-two row scales, three FP16 MFMA
+                           products (hi hi + hi lo + lo hi), FP32 accumulate:
+                           an FP32-accurate update on the FP16 matrix cores */
 };
 enum {
     RLG_REFINE_IR = 0,     /* classical iterative refinement */

@@ -524,9 +524,269 @@ __global__ __launch_bounds__(GTHREADS) void k_gemm_nt_lowp(
     epilogue_f32(acc, C, ldc, M, N, m0 + wm, n0 + wn, lr, lk, alpha, beta);
 }
 
+// ---------------------------------------------------------------- split FP16
+//
+// FP32 products on the FP16 matrix cores: every FP32 operand x is scaled by
+// a power of two per row (so that the row's largest |x| lands in
+// [2^13, 2^14)) and split as x = hi + lo, hi = fp16(x), lo = fp16(x - hi);
+// the product accumulates hi_a hi_b + hi_a lo_b + lo_a hi_b in FP32 (the
+// dropped lo_a lo_b is below 2^-22 of the scaled magnitudes), and the
+// epilogue multiplies by the two inverse scales.  The result is as accurate
+// as an FP32 product (normwise error of order 2^-22 |A||B| plus FP32
+// accumulation) at three FP16 MFMAs per k-step.
+
+constexpr int SBK = 32;  // k per stage of the split kernel (four operands staged)
+
+// Power-of-two scale for a row whose largest |x| is mx, and its inverse.
+__device__ __forceinline__ void split_scale(float mx, float &sc, float &inv) {
+    if (!(mx > 0.f) || !isfinite(mx)) { sc = inv = 1.f; return; }
+    int e;
+    frexpf(mx, &e);                      // mx = f 2^e, f in [0.5, 1)
+    e = e < -100 ? -100 : (e > 100 ? 100 : e);
+    sc = ldexpf(1.f, 14 - e);
+    inv = ldexpf(1.f, e - 14);
+}
+
+__device__ __forceinline__ void split_one(float x, float sc, _Float16 &hi, _Float16 &lo) {
+    const float y = x * sc;
+    hi = (_Float16)y;
+    lo = (_Float16)(y - (float)hi);
+}
+
+// dst_{hi,lo}[i][k] from the rows of src (rows x K, k < K; zero up to Kp),
+// one wave per row; inv[i] = the row's inverse scale.
+__global__ __launch_bounds__(256) void k_pack_rows_split(const float *__restrict__ src, long lds, int rows,
+                                                         int K, int Kp, _Float16 *__restrict__ hi,
+                                                         _Float16 *__restrict__ lo, long ldd,
+                                                         float *__restrict__ inv) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= rows) return;
+    const float *r = src + (long)i * lds;
+    float mx = 0.f;
+    for (int k = lane; k < K; k += 64) mx = fmaxf(mx, fabsf(r[k]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    float sc, iv;
+    split_scale(mx, sc, iv);
+    if (lane == 0) inv[i] = iv;
+    typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+    for (int k0 = lane * 8; k0 < Kp; k0 += 64 * 8) {
+        h8v vh, vl;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            _Float16 a, b;
+            split_one(k0 + q < K ? r[k0 + q] : 0.f, sc, a, b);
+            vh[q] = a;
+            vl[q] = b;
+        }
+        *reinterpret_cast<h8v *>(hi + (long)i * ldd + k0) = vh;
+        *reinterpret_cast<h8v *>(lo + (long)i * ldd + k0) = vl;
+    }
+}
+
+// dst_{hi,lo}[j][k] = split(src[k][j]) for a K x cols source, 64 columns per
+// workgroup: the column maxima over all K first, then 64 x 64 tiles through
+// LDS (coalesced reads along j, writes along k).
+__global__ __launch_bounds__(256) void k_pack_cols_split(const float *__restrict__ src, long lds, int K,
+                                                         int cols, int Kp, _Float16 *__restrict__ hi,
+                                                         _Float16 *__restrict__ lo, long ldd,
+                                                         float *__restrict__ inv) {
+    __shared__ float t[64][65];
+    __shared__ float cm[4][64];
+    __shared__ float csc[64];
+    const int j0 = blockIdx.x * 64;
+    const int jj = threadIdx.x & 63, kq = threadIdx.x >> 6;
+    {
+        float mx = 0.f;
+        const int j = j0 + jj;
+        if (j < cols)
+            for (int k = kq; k < K; k += 4) mx = fmaxf(mx, fabsf(src[(long)k * lds + j]));
+        cm[kq][jj] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const float mx = fmaxf(fmaxf(cm[0][jj], cm[1][jj]), fmaxf(cm[2][jj], cm[3][jj]));
+        float sc, iv;
+        split_scale(mx, sc, iv);
+        csc[jj] = sc;
+        if (j0 + jj < cols) inv[j0 + jj] = iv;
+    }
+    typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+    for (int k0 = 0; k0 < Kp; k0 += 64) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int kk = e >> 6, c = e & 63;
+            const int k = k0 + kk, j = j0 + c;
+            t[kk][c] = (k < K && j < cols) ? src[(long)k * lds + j] : 0.f;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 8; e += 256) {
+            const int c = e >> 3, kc = (e & 7) * 8;
+            const int j = j0 + c;
+            if (j >= cols) continue;
+            h8v vh, vl;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                _Float16 a, b;
+                split_one(t[kc + q][c], csc[c], a, b);
+                vh[q] = a;
+                vl[q] = b;
+            }
+            *reinterpret_cast<h8v *>(hi + (long)j * ldd + k0 + kc) = vh;
+            *reinterpret_cast<h8v *>(lo + (long)j * ldd + k0 + kc) = vl;
+        }
+    }
+}
+
+// C = alpha * (Ah Bh^T + Ah Bl^T + Al Bh^T) diag-scaled + beta * C with the
+// split operands (M x Kp and N x Kp, Kp a multiple of 64): 128 x 128 tile per
+// workgroup, 2 x 2 waves of 64 x 64, four operands staged 32 k at a time
+// through double-buffered LDS with 16-byte moves.
+template <bool LOWER>
+__global__ __launch_bounds__(GTHREADS, 2) void k_gemm_nt_split(
+    int M, int N, int Kp, float alpha, const _Float16 *__restrict__ Ah, const _Float16 *__restrict__ Al,
+    const float *__restrict__ ia, const _Float16 *__restrict__ Bh, const _Float16 *__restrict__ Bl,
+    const float *__restrict__ ib, long ldk, float beta, float *__restrict__ C, long ldc, int tiles_m,
+    int tiles_n) {
+    using V = h8;
+    __shared__ __attribute__((aligned(16))) _Float16 S[2][4][GBM][SBK + LPAD];
+
+    int tm, tn;
+    tile_coords<LOWER>(tiles_m, tiles_n, tm, tn);
+    if (LOWER && tn > tm) return;
+    const int m0 = tm * GBM, n0 = tn * GBN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4{0, 0, 0, 0};
+
+    // a stage is 128 rows x 32 k per operand = 512 chunks of 8 halves: 2 per thread
+    uint4 rg[4][2];
+    const _Float16 *src[4] = {Ah, Al, Bh, Bl};
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int o = 0; o < 4; ++o)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int c = q * GTHREADS + tid;
+                const int row = c >> 2, kc = (c & 3) * 8;
+                const int g = (o < 2 ? m0 : n0) + row;
+                const bool in = g < (o < 2 ? M : N);
+                rg[o][q] = in ? *reinterpret_cast<const uint4 *>(src[o] + (long)g * ldk + k0 + kc)
+                              : uint4{0, 0, 0, 0};
+            }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int o = 0; o < 4; ++o)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int c = q * GTHREADS + tid;
+                const int row = c >> 2, kc = (c & 3) * 8;
+                *reinterpret_cast<uint4 *>(&S[buf][o][row][kc]) = rg[o][q];
+            }
+    };
+
+    const int KT = Kp / SBK;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < KT) load((kt + 1) * SBK);
+        V ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            ah[i] = *reinterpret_cast<const V *>(&S[buf][0][wm + i * 16 + lr][lk * 8]);
+            al[i] = *reinterpret_cast<const V *>(&S[buf][1][wm + i * 16 + lr][lk * 8]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            bh[j] = *reinterpret_cast<const V *>(&S[buf][2][wn + j * 16 + lr][lk * 8]);
+            bl[j] = *reinterpret_cast<const V *>(&S[buf][3][wn + j * 16 + lr][lk * 8]);
+        }
+        // the small cross terms first, then the leading one
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            }
+        if (kt + 1 < KT) store(buf ^ 1);
+        __syncthreads();
+    }
+
+    // undo the row and column scales (powers of two: exact)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gj = n0 + wn + j * 16 + lr;
+            const float sj = gj < N ? ib[gj] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wm + i * 16 + lk * 4 + r;
+                const float si = gi < M ? ia[gi] : 0.f;
+                acc[i][j][r] = acc[i][j][r] * si * sj;
+            }
+        }
+    epilogue_f32(acc, C, ldc, M, N, m0 + wm, n0 + wn, lr, lk, alpha, beta);
+}
+
+static size_t al256z(size_t x) { return (x + 255) & ~(size_t)255; }
+
 size_t gemm_lowp_pack_bytes(size_t m, size_t n, size_t k) {
+    // room for the split form: hi and lo copies of both operands and their
+    // inverse scales (the plain FP16/BF16 form uses the first half)
     const size_t kp = (k + PKB - 1) / PKB * PKB;
-    return ((m * kp * 2 + 255) & ~(size_t)255) + ((n * kp * 2 + 255) & ~(size_t)255);
+    return 2 * al256z(m * kp * 2) + 2 * al256z(n * kp * 2) + al256z(m * 4) + al256z(n * 4);
+}
+
+static int gemm_split_packed(hipStream_t s, size_t m, size_t n, size_t k, float alpha, const float *a,
+                             size_t lda, const float *b, size_t ldb, float beta, float *c, size_t ldc,
+                             bool trans_b, bool lower_only, void *pack) {
+    const size_t kp = (k + PKB - 1) / PKB * PKB;
+    char *p = static_cast<char *>(pack);
+    _Float16 *ah = reinterpret_cast<_Float16 *>(p); p += al256z(m * kp * 2);
+    _Float16 *al = reinterpret_cast<_Float16 *>(p); p += al256z(m * kp * 2);
+    _Float16 *bh = reinterpret_cast<_Float16 *>(p); p += al256z(n * kp * 2);
+    _Float16 *bl = reinterpret_cast<_Float16 *>(p); p += al256z(n * kp * 2);
+    float *ia = reinterpret_cast<float *>(p); p += al256z(m * 4);
+    float *ib = reinterpret_cast<float *>(p);
+    {
+        RLG_PROF_W("pack_split", s, 4.0 * (double)(m + n) * k + 4.0 * (double)(m + n) * kp);
+        hipLaunchKernelGGL(k_pack_rows_split, dim3(ceil_div(m, 4)), dim3(256), 0, s, a, (long)lda, (int)m,
+                           (int)k, (int)kp, ah, al, (long)kp, ia);
+        RLG_LAUNCHED();
+        if (trans_b)
+            hipLaunchKernelGGL(k_pack_rows_split, dim3(ceil_div(n, 4)), dim3(256), 0, s, b, (long)ldb,
+                               (int)n, (int)k, (int)kp, bh, bl, (long)kp, ib);
+        else
+            hipLaunchKernelGGL(k_pack_cols_split, dim3(ceil_div(n, 64)), dim3(256), 0, s, b, (long)ldb,
+                               (int)k, (int)n, (int)kp, bh, bl, (long)kp, ib);
+        RLG_LAUNCHED();
+    }
+    const int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
+    const double flops = (lower_only ? 1.0 : 2.0) * (double)m * (double)n * (double)k;
+    RLG_PROF_W("gemm_f16x3", s, flops);
+    if (lower_only)
+        hipLaunchKernelGGL(k_gemm_nt_split<true>, dim3(tiles_m * tiles_n), dim3(GTHREADS), 0, s, (int)m,
+                           (int)n, (int)kp, alpha, ah, al, ia, bh, bl, ib, (long)kp, beta, c, (long)ldc,
+                           tiles_m, tiles_n);
+    else
+        hipLaunchKernelGGL(k_gemm_nt_split<false>, dim3(tiles_m * tiles_n), dim3(GTHREADS), 0, s, (int)m,
+                           (int)n, (int)kp, alpha, ah, al, ia, bh, bl, ib, (long)kp, beta, c, (long)ldc,
+                           tiles_m, tiles_n);
+    RLG_LAUNCHED();
+    return RLG_OK;
 }
 
 template <class H>
@@ -631,6 +891,13 @@ int gemm_lowp(hipStream_t s, int prec, size_t m, size_t n, size_t k, float alpha
               size_t ldc, bool trans_b, bool lower_only, void *pack, size_t pack_bytes) {
     if (m == 0 || n == 0) return RLG_OK;
     RLG_TRY(check_dims(m, n, k));
+    if (prec == RLG_PREC_F16X3) {
+        if (k == 0 || !pack || pack_bytes < gemm_lowp_pack_bytes(m, n, k)) {
+            set_error("gemm_lowp: the split FP16 form needs its packed-operand area");
+            return RLG_E_INVALID_ARG;
+        }
+        return gemm_split_packed(s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, trans_b, lower_only, pack);
+    }
     if (k > 0 && pack && pack_bytes >= gemm_lowp_pack_bytes(m, n, k)) {
         if (prec == RLG_PREC_F16)
             return gemm_lowp_packed<_Float16>(s, "gemm_f16", m, n, k, alpha, a, lda, b, ldb, beta, c,

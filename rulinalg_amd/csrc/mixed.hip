@@ -439,7 +439,7 @@ int rlg_solve_mixed_f64_dev(size_t n, const double *a, size_t lda, size_t nrhs, 
     if (lda < n || ldb < n || ldx < n) { set_error("bad leading dimension"); return RLG_E_INVALID_ARG; }
     rlg_mixed_opts o;
     if (opts) o = *opts; else rlg_mixed_opts_default(&o);
-    if (o.factor_prec < RLG_PREC_F64 || o.factor_prec > RLG_PREC_BF16) {
+    if (o.factor_prec < RLG_PREC_F64 || o.factor_prec > RLG_PREC_F16X3) {
         set_error("unknown factor precision %d", o.factor_prec);
         return RLG_E_INVALID_ARG;
     }
