@@ -184,6 +184,23 @@ int rlg_lu_det_f64(const rlg_lu *lu, double *det);
 int rlg_lu_inverse_f64(const rlg_lu *lu, double *inv, size_t ld);
 void rlg_lu_free(rlg_lu *lu);
 
+/* In-place FP64 LU factorization of a device matrix that allocates no device
+ * memory.  No counterpart in the reference: it is for callers that cannot
+ * call hipMalloc (code under HIP graph capture, serving loops and harnesses
+ * that forbid allocation).  The factorization is rlg_lu_decompose_f64_dev's
+ * (same kernels, same packed factor and pivots); its scratch is a static
+ * device array of RLG_LU_NOALLOC_SCRATCH bytes that each GPU holds until
+ * the process ends.  ipiv (n device int64) receives the step pivots as
+ * rlg_lu_ipiv reports them; *info (device int64) receives n on success, the
+ * first step whose pivot was below epsilon, or -1 if the panel exchange gave
+ * up waiting.  Asynchronous on `stream` (NULL: the library's stream).  Every
+ * call on one GPU shares that GPU's scratch: calls that overlap on one GPU
+ * race.  A call whose scratch would exceed the array returns
+ * RLG_E_INVALID_ARG. */
+#define RLG_LU_NOALLOC_SCRATCH (1u << 20)
+int rlg_lu_factor_f64_noalloc(size_t n, double *a, size_t lda, int64_t *ipiv,
+                              int64_t *info, void *stream);
+
 /* ---------------------------------------------------------------- FullPivLu
  * FullPivLu (src/matrix/decomposition/lu.rs:339-599): P A Q = L U with
  * complete pivoting.  The packed factor, p and q are identical to the

@@ -105,6 +105,7 @@ SIGNATURES = {
     "rlg_permute_rows_f64_dev": (i32, [sz, vp, sz, vp, sz, vp, sz, vp]),
     "rlg_permute_cols_f64_dev": (i32, [sz, sz, vp, vp, sz, vp, sz, vp]),
     "rlg_lu_free": (None, [vp]),
+    "rlg_lu_factor_f64_noalloc": (i32, [sz, vp, sz, vp, vp, vp]),
     "rlg_fplu_decompose_f64": (i32, [sz, sz, vp, sz, C.POINTER(vp)]),
     "rlg_fplu_decompose_f32": (i32, [sz, sz, vp, sz, C.POINTER(vp)]),
     "rlg_fplu_decompose_f64_dev": (i32, [sz, vp, sz, vp, C.POINTER(vp)]),
