@@ -46,7 +46,7 @@ def parse():
     p.add_argument("--n", type=int, default=16384)
     p.add_argument("--nrhs", type=int, default=1)
     p.add_argument("--kappa", type=float, default=1e4)
-    p.add_argument("--prec", default="f32", choices=sorted(PREC))
+    p.add_argument("--prec", default="f16x3", choices=sorted(PREC))
     p.add_argument("--pivot", default="partial", choices=sorted(PIVOT),
                    help="pivoting of the low-precision factorization")
     p.add_argument("--plain-steps", type=int, default=None)
@@ -313,9 +313,11 @@ def main():
         B32 = torch.randn(32, n, dtype=torch.float64, device=dev, generator=g)
         X32 = torch.empty_like(B32)
         torch.cuda.synchronize()
-        for prec in ("f32", "f16"):
+        for prec in ("f16x3", "f32", "f16"):
             mixed_once(A, B32, X32, 32, prec)
             variants[f"mixed_{prec}_32rhs_kappa1e4"] = mixed_once(A, B32, X32, 32, prec)
+        mixed_once(A, B, X, nrhs, "f32")
+        variants["mixed_f32_1rhs_kappa1e4"] = mixed_once(A, B, X, nrhs, "f32")
         # plain FP64 LU + 16 right-hand sides (factor and solve)
         B16, X16 = B32[:16].contiguous(), X32[:16].contiguous()
         assert lib.rlg_dev_copy(work.data_ptr(), A.data_ptr(), A.numel() * 8, sptr) == 0
@@ -333,19 +335,21 @@ def main():
         variants["plain_lu_16rhs"] = {"ms": round(ms, 3), "tflops": round(fl / (ms / 1e3) / 1e12, 2),
                                       "frac_fp64_peak": round(fl / (ms / 1e3) / 78.6e12, 4),
                                       "backward_error": eta.value}
-        # kappa 1e8: refinement from the low-precision factors stalls and the
-        # FP64 factorization takes over
+        # kappa 1e8 (the configuration's second run; at this order the
+        # refinement still converges) and kappa 1e12 (refinement from
+        # FP32-accurate factors stalls and the FP64 factorization takes over)
         if uq is not None:
-            s8 = torch.logspace(0.0, 8.0, n, dtype=torch.float64, device=dev)
-            us8 = uq * s8.unsqueeze(0)
-            A8 = torch.empty_like(A)
-            torch.cuda.synchronize()
-            assert lib.rlg_gemm_f64_dev(n, n, n, 1.0, us8.data_ptr(), n, vt.data_ptr(), n, 0.0,
-                                        A8.data_ptr(), n, sptr) == 0
-            assert lib.rlg_synchronize() == 0
-            del us8
-            variants[f"mixed_{args.prec}_1rhs_kappa1e8"] = mixed_once(A8, B, X, nrhs, args.prec)
-            del A8
+            for lk in (8, 12):
+                sk = torch.logspace(0.0, float(lk), n, dtype=torch.float64, device=dev)
+                usk = uq * sk.unsqueeze(0)
+                Ak = torch.empty_like(A)
+                torch.cuda.synchronize()
+                assert lib.rlg_gemm_f64_dev(n, n, n, 1.0, usk.data_ptr(), n, vt.data_ptr(), n, 0.0,
+                                            Ak.data_ptr(), n, sptr) == 0
+                assert lib.rlg_synchronize() == 0
+                del usk
+                variants[f"mixed_{args.prec}_1rhs_kappa1e{lk}"] = mixed_once(Ak, B, X, nrhs, args.prec)
+                del Ak
         for k_, v_ in variants.items():
             log(f"{k_}: {v_}")
         result["variants"] = variants

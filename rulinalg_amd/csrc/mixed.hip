@@ -425,7 +425,7 @@ extern "C" {
 
 void rlg_mixed_opts_default(rlg_mixed_opts *o) {
     if (!o) return;
-    o->factor_prec = RLG_PREC_F32;
+    o->factor_prec = RLG_PREC_F16X3;
     o->refine = RLG_REFINE_IR;
     o->tol = 1e-13;
     o->max_iter = 30;

@@ -126,7 +126,7 @@ int main() {
     rlg_mixed_opts o;
     rlg_mixed_opts_default(&o);
     rlg_mixed_opts_default(nullptr);
-    EXPECT(o.factor_prec == RLG_PREC_F32 && o.refine == RLG_REFINE_IR && o.tol == 1e-13, "mixed defaults");
+    EXPECT(o.factor_prec == RLG_PREC_F16X3 && o.refine == RLG_REFINE_IR && o.tol == 1e-13, "mixed defaults");
     rlg_mixed_report rep;
     EXPECT(rlg_solve_mixed_f64(3, a.data(), 2, 1, b.data(), 3, c.data(), 3, &o, &rep) == RLG_E_INVALID_ARG,
            "mixed lda");

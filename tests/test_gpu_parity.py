@@ -479,12 +479,14 @@ def test_mixed_refined_well_conditioned(prec):
         assert eta(a, x[i], b[i]) <= 1e-13
 
 
-@pytest.mark.parametrize("prec", [_lib.PREC_F32, _lib.PREC_F16])
-@pytest.mark.parametrize("n", [200, 1500])
+@pytest.mark.parametrize("prec,n", [(_lib.PREC_F32, 200), (_lib.PREC_F32, 1500), (_lib.PREC_F32, 4100),
+                                    (_lib.PREC_F16, 200), (_lib.PREC_F16, 1500),
+                                    (_lib.PREC_F16X3, 1500), (_lib.PREC_F16X3, 4100)])
 def test_mixed_tournament_pivoting(prec, n):
-    """TSLU panels (n = 1500: twelve 128-row chunks, a four-level tree):
-    refinement must converge without the FP64 fallback, to the same bar as
-    partial pivoting."""
+    """TSLU panels (256-row leaves merged four at a time: n = 1500 is a
+    three-level tree, n = 4100 a four-level one whose last panel is 4 columns
+    wide): refinement must converge without the FP64 fallback, to the same
+    bar as partial pivoting."""
     a = wl.conditioned(n, 1e3, 5, 6)
     b = wl.gaussian(2, n, 7)
     x, rep = rl.solve_mixed(a, b, rl.MixedOptions(factor_prec=prec, pivot=_lib.PIVOT_TOURNAMENT))
