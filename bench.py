@@ -61,6 +61,10 @@ def parse():
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run three device copies of MIB MiB (the plain copy kernel that "
                         "calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    p.add_argument("--serve-slots", type=int, default=4,
+                   help="slot workers (streams) of the serving loop for systems above order 512")
+    p.add_argument("--serving-only", action="store_true",
+                   help="only run the serving traces (2,000 and 200 requests/s) and exit")
     p.add_argument("--cholesky-only", action="store_true",
                    help="only time Cholesky::decompose at n=16384 (two factorizations) and exit")
     return p.parse_args()
@@ -101,6 +105,11 @@ def main():
     dev = torch.device("cuda:0")
     if args.cholesky_only:
         print(json.dumps(cholesky_16384(lib, torch, dev, None)), flush=True)
+        return
+    if args.serving_only:
+        print(json.dumps({"offered_2000_per_s": serving_trace(lib, torch, dev, 2000, 1.0, 50, args.serve_slots),
+                          "offered_200_per_s": serving_trace(lib, torch, dev, 200, 2.0, 51, args.serve_slots)}),
+              flush=True)
         return
     # torch prepares the inputs on its own stream and is synchronized before
     # any library call; every library call runs on the library's stream
@@ -358,8 +367,8 @@ def main():
         result.update(extras(lib, torch, dev, sptr))
         result["batched"] = batched(lib, torch, dev)
         # the serving loop at the configured rate and at a rate one GPU sustains
-        result["serving"] = {"offered_2000_per_s": serving_trace(lib, torch, dev, 2000, 1.0, 50),
-                             "offered_200_per_s": serving_trace(lib, torch, dev, 200, 2.0, 51)}
+        result["serving"] = {"offered_2000_per_s": serving_trace(lib, torch, dev, 2000, 1.0, 50, args.serve_slots),
+                             "offered_200_per_s": serving_trace(lib, torch, dev, 200, 2.0, 51, args.serve_slots)}
 
     # ------------------------------------------------------------ CPU baseline
     if not args.no_cpu:
@@ -537,14 +546,15 @@ def batched(lib, torch, dev, count=20000, nmin=8, nmax=512, seed=30):
     return out
 
 
-def serving_trace(lib, torch, dev, rate, seconds, seed):
+def serving_trace(lib, torch, dev, rate, seconds, seed, slots=4):
     """Serving workload: Poisson arrivals at `rate` requests/s for `seconds`;
     n log-uniform on [16, 4096]; 70 % LU (Gaussian) / 30 % Cholesky
     (G G^T / n + I); one right-hand side; about 10 % of the requests keep
     their factor resident and 20 % re-solve a new right-hand side against an
     earlier kept factor; 64 GB admission budget.  Inputs are generated on the
     device before the clock starts (as if uploaded on arrival).  Each trace
-    runs plain FP64 and mixed (FP32-factor + FP64 refinement)."""
+    runs plain FP64 and mixed (split-FP16 factor + FP64 refinement) with
+    `slots` slot workers for the systems above order 512."""
     from rulinalg_amd import serving
     rng = np.random.default_rng(seed)
     count = int(rate * seconds)
@@ -569,14 +579,16 @@ def serving_trace(lib, torch, dev, rate, seconds, seed):
     for sp in specs:
         sp["b"] = torch.randn(sp["n"], dtype=torch.float64, device=dev, generator=g)
     torch.cuda.synchronize()
-    out = {"rate_per_s": rate, "requests": count, "seconds_of_arrivals": seconds}
+    out = {"rate_per_s": rate, "requests": count, "seconds_of_arrivals": seconds, "slots": slots}
     for mixed in (False, True):
         reqs = [serving.Request(i, sp["kind"], sp["n"], 1, sp["a"], sp["b"], torch.empty_like(sp["b"]),
                                 float(arrivals[i]), keep_id=sp.get("keep"), reuse_id=sp.get("reuse"))
                 for i, sp in enumerate(specs)]
-        loop = serving.ServingLoop(lib, torch, dev, mem_budget_bytes=64 << 30, mixed=mixed)
+        loop = serving.ServingLoop(lib, torch, dev, mem_budget_bytes=64 << 30, mixed=mixed, slots=slots)
         loop.run(reqs)
         loop.close()
+        del loop
+        torch.cuda.synchronize()
         s = serving.summarize(reqs)
         out["mixed" if mixed else "plain"] = s
         log(f"serving {rate}/s {'mixed' if mixed else 'plain'}: {s}")
@@ -731,3 +743,12 @@ def cpu_baseline(cpu_n, n):
 
 if __name__ == "__main__":
     main()
+    # The result line is printed and flushed.  The process then ends without
+    # running the interpreter's finalizers: after the serving loop's worker
+    # threads have driven their own HIP streams, the teardown of the HIP
+    # runtime and PyTorch once ended in a segmentation fault after every
+    # result had been printed, which would turn a complete run into a failed
+    # exit status.
+    sys.stdout.flush()
+    sys.stderr.flush()
+    os._exit(0)

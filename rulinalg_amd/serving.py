@@ -1,34 +1,46 @@
 """Continuous-batching request loop over the batched and per-system solvers.
 
-One host thread drives the library's stream.  Every pass of the loop admits,
-in arrival order, each request that has arrived by now and whose device bytes
-fit under the memory budget together with the resident factors (the
-least-recently-used resident factor that no admitted request needs is evicted
-to make room; a request that can never fit fails with RLG_E_OOM).  The
-admitted requests then run:
+A dispatcher (the thread that calls ``run``) admits requests in arrival order
+as they arrive, while earlier ones are still on the GPU: a request is
+admitted when its device bytes fit under the memory budget together with the
+bytes of the requests in flight and of the resident factors (the
+least-recently-used resident factor that no admitted request needs is
+evicted to make room; a request that can never fit fails with RLG_E_OOM).
+Admitted requests go to workers, each a host thread with its own HIP stream:
 
-* one-shot systems of order <= 512 together in ONE persistent batched launch
-  (``rlg_batch_solve_f64_dev``, or ``rlg_batch_solve_mixed_dev`` in mixed
-  mode): their work overlaps on the device;
-* larger systems, and requests that keep or reuse a resident factor, one at a
-  time through the per-system path (``rlg_{lu,chol}_decompose_f64_dev`` +
-  ``_solve_f64_dev``, or ``rlg_solve_mixed_f64_dev`` for one-shot mixed
-  requests).
+* one-shot systems of order <= 512 collect in a queue that the batch worker
+  empties whenever it is idle: all of them run in ONE persistent batched
+  launch (``rlg_batch_solve_f64_dev``, or ``rlg_batch_solve_mixed_dev`` in
+  mixed mode), so their work overlaps on the device, and systems arriving
+  meanwhile form the next launch;
+* larger systems, and requests that keep or reuse a resident factor, go to
+  the first free slot worker, which runs the per-system path
+  (``rlg_{lu,chol}_decompose_f64_dev`` + ``_solve_f64_dev``, or
+  ``rlg_solve_mixed_f64_dev`` for one-shot mixed LU) on its stream; the slot
+  workers' systems run on the device at the same time.
 
-Requests that arrive while a pass runs are admitted by the next pass.  A
-request that keeps its factor (``keep_id``) leaves the handle resident under
-that id, with its original matrix for the backward error; a later request with
-``reuse_id`` solves new right-hand sides against it without refactoring.
+Every call a worker makes is ordered on its own stream and a worker waits
+only for its own stream, so no request waits for another's work to finish;
+a request's latency ends when its worker's stream has delivered the
+solution.  A request that keeps its factor (``keep_id``) leaves the handle
+resident under that id, with its original matrix for the backward error; a
+later request with ``reuse_id`` solves new right-hand sides against it
+without refactoring (several such solves may run at once: the handle is
+immutable).  A reuse request that arrives while the factor it names is
+still being computed waits for it.
 
 The reference has no serving layer (rulinalg is a library of synchronous
 calls); the loop serves its calls: PartialPivLu::decompose + solve
-(src/matrix/decomposition/lu.rs:163-244) and Cholesky::decompose + solve
-(src/matrix/decomposition/cholesky.rs:116-202).  Results do not depend on what
-else is in the pass: the batched kernel solves every system in one workgroup
-with a fixed operation order, and the per-system path is deterministic.
+(src/matrix/decomposition/lu.rs:163-244, solves on `&self` at lu.rs:231) and
+Cholesky::decompose + solve (src/matrix/decomposition/cholesky.rs:116-202,
+cholesky.rs:194).  Results do not depend on what else is in flight: the
+batched kernel solves every system in one workgroup with a fixed operation
+order, and the per-system path is deterministic.
 """
 import collections
 import ctypes as C
+import queue
+import threading
 import time
 
 from . import _lib
@@ -44,7 +56,7 @@ class Request:
     request); ``x`` (nrhs*n) receives the solution."""
 
     __slots__ = ("rid", "kind", "n", "nrhs", "a", "b", "x", "arrival", "keep_id", "reuse_id",
-                 "check", "status", "done", "eta", "path", "fell_back", "t_arr")
+                 "check", "status", "done", "eta", "path", "fell_back", "t_arr", "need", "_kept")
 
     def __init__(self, rid, kind, n, nrhs, a, b, x, arrival, keep_id=None, reuse_id=None, check=True):
         self.rid, self.kind, self.n, self.nrhs = rid, kind, n, nrhs
@@ -52,20 +64,59 @@ class Request:
         self.keep_id, self.reuse_id, self.check = keep_id, reuse_id, check
         self.status, self.done, self.eta, self.path, self.fell_back = None, None, None, None, 0
         self.t_arr = None  # arrival on the host clock, set by ServingLoop.run
+        self.need = 0      # device bytes held while in flight
+        self._kept = None  # (handle, factor tensor) of a kept factor, for the dispatcher
 
     @property
     def latency_ms(self):
         return None if self.done is None else (self.done - self.t_arr) * 1e3
 
 
+class _Worker(threading.Thread):
+    """A host thread with its own stream that runs the jobs it is given, one
+    at a time, and reports each finished job on the completion queue."""
+
+    def __init__(self, loop, name):
+        super().__init__(name=name, daemon=True)
+        self.loop = loop
+        self.jobs = queue.Queue()
+        self.stream = loop._new_stream()
+        self.busy = False
+
+    def run(self):
+        while True:
+            job = self.jobs.get()
+            if job is None:
+                return
+            fn, arg = job
+            try:
+                fn(self, arg)
+                err = None
+            except Exception as e:  # reported to the dispatcher, which re-raises it
+                err = e
+            self.loop.completions.put((self, arg, err))
+
+
 class ServingLoop:
-    def __init__(self, lib, torch, dev, mem_budget_bytes=64 << 30, mixed=False, tol=1e-13):
+    def __init__(self, lib, torch, dev, mem_budget_bytes=64 << 30, mixed=False, tol=1e-13, slots=4):
         self.lib, self.torch, self.dev = lib, torch, dev
         self.budget = mem_budget_bytes
         self.mixed = mixed
         self.tol = tol
         self.resident = collections.OrderedDict()  # id -> (handle, kind, n, work, a, bytes)
         self.resident_bytes = 0
+        self.completions = queue.Queue()
+        self.batch_worker = _Worker(self, "rlg-batch")
+        self.slot_workers = [_Worker(self, f"rlg-slot{i}") for i in range(max(1, slots))]
+        for w in [self.batch_worker] + self.slot_workers:
+            w.start()
+
+    # ---------------------------------------------------------------- streams
+    def _new_stream(self):
+        return self.torch.cuda.Stream(device=self.dev)
+
+    def _sptr(self, worker):
+        return C.c_void_p(worker.stream.cuda_stream)
 
     # ---------------------------------------------------------------- memory
     @staticmethod
@@ -74,7 +125,7 @@ class ServingLoop:
         batch's copies of A and B), plus what stays resident if kept."""
         if r.reuse_id is not None:
             return 0
-        return 8 * r.n * r.n + 8 * r.nrhs * r.n
+        return 8 * r.n * r.n + 8 * r.nrhs * r.n + (16 * r.n * r.n if r.keep_id is not None else 0)
 
     def _evict_for(self, need, protect):
         while self.resident_bytes + need > self.budget:
@@ -94,105 +145,106 @@ class ServingLoop:
             self._drop(fid)
 
     def close(self):
+        """Frees the resident factors and stops the worker threads (the loop
+        cannot run afterwards)."""
         for fid in list(self.resident):
             self._drop(fid)
+        for w in [self.batch_worker] + self.slot_workers:
+            w.jobs.put(None)
+        for w in [self.batch_worker] + self.slot_workers:
+            w.join(timeout=60)
 
-    # ---------------------------------------------------------------- passes
-    def _eta(self, n, a, nrhs, b, x):
+    # ---------------------------------------------------------------- jobs
+    def _eta(self, worker, n, a, nrhs, b, x):
         e = C.c_double()
-        st = self.lib.rlg_backward_error_f64_dev(n, a, n, nrhs, b, n, x, n, C.byref(e), None)
+        st = self.lib.rlg_backward_error_f64_dev(n, a, n, nrhs, b, n, x, n, C.byref(e), self._sptr(worker))
         return e.value if st == 0 else float("inf")
 
-    def _run_batch(self, reqs):
+    def _job_batch(self, worker, reqs):
         torch, lib = self.torch, self.lib
         count = len(reqs)
         items = (_lib.BatchItem * count)()
-        # plain: the kernel factors in place, so it works on a copy of A; the
-        # solution overwrites the copy of B in both forms
-        bcat = torch.cat([r.b for r in reqs])
-        acat = torch.cat([r.a for r in reqs]) if not self.mixed else None
-        piv = torch.empty(sum(r.n for r in reqs), dtype=torch.int64, device=self.dev)
-        boff = aoff = poff = 0
-        for i, r in enumerate(reqs):
-            pa = r.a.data_ptr() if self.mixed else acat.data_ptr() + 8 * aoff
-            items[i] = _lib.BatchItem(r.kind, r.n, r.nrhs, -1, pa, bcat.data_ptr() + 8 * boff,
-                                      piv.data_ptr() + 8 * poff)
-            aoff += r.n * r.n
-            boff += r.nrhs * r.n
-            poff += r.n
-        dev_items = torch.empty(C.sizeof(items), dtype=torch.uint8, device=self.dev)
-        torch.cuda.synchronize()  # torch's copies are on its own stream
-        assert lib.rlg_dev_upload(dev_items.data_ptr(), C.addressof(items), C.sizeof(items)) == 0
-        fb = C.c_size_t(0)
-        if self.mixed:
-            st = lib.rlg_batch_solve_mixed_dev(count, dev_items.data_ptr(), 30, self.tol, C.byref(fb), None)
-        else:
-            st = lib.rlg_batch_solve_f64_dev(count, dev_items.data_ptr(), None)
-        if st != 0:
-            raise RuntimeError(_lib.last_error())
-        assert lib.rlg_synchronize() == 0
+        sp = self._sptr(worker)
+        with torch.cuda.stream(worker.stream):
+            # plain: the kernel factors in place, so it works on a copy of A;
+            # the solution overwrites the copy of B in both forms
+            bcat = torch.cat([r.b for r in reqs])
+            acat = torch.cat([r.a for r in reqs]) if not self.mixed else None
+            piv = torch.empty(sum(r.n for r in reqs), dtype=torch.int64, device=self.dev)
+            boff = aoff = poff = 0
+            for i, r in enumerate(reqs):
+                pa = r.a.data_ptr() if self.mixed else acat.data_ptr() + 8 * aoff
+                items[i] = _lib.BatchItem(r.kind, r.n, r.nrhs, -1, pa, bcat.data_ptr() + 8 * boff,
+                                          piv.data_ptr() + 8 * poff)
+                aoff += r.n * r.n
+                boff += r.nrhs * r.n
+                poff += r.n
+            host = torch.frombuffer(bytearray(C.string_at(C.addressof(items), C.sizeof(items))),
+                                    dtype=torch.uint8)
+            dev_items = host.to(self.dev)
+            fb = C.c_size_t(0)
+            if self.mixed:
+                st = lib.rlg_batch_solve_mixed_dev(count, dev_items.data_ptr(), 30, self.tol, C.byref(fb), sp)
+            else:
+                st = lib.rlg_batch_solve_f64_dev(count, dev_items.data_ptr(), sp)
+            if st != 0:
+                raise RuntimeError(_lib.last_error())
+            boff = 0
+            for r in reqs:
+                r.x.copy_(bcat[boff:boff + r.nrhs * r.n])
+                boff += r.nrhs * r.n
+            back = dev_items.cpu()  # waits for this worker's stream only
         t = time.perf_counter()
-        assert lib.rlg_dev_download(C.addressof(items), dev_items.data_ptr(), C.sizeof(items)) == 0
-        boff = 0
+        C.memmove(C.addressof(items), back.numpy().ctypes.data, C.sizeof(items))
         for i, r in enumerate(reqs):
-            sol = bcat[boff:boff + r.nrhs * r.n]
-            boff += r.nrhs * r.n
             r.status = items[i].status
             r.done = t
             r.path = "batched_mixed" if self.mixed else "batched"
-            r.x.copy_(sol)
-        torch.cuda.synchronize()
         for r in reqs:
             if r.check and r.status == 0:
-                r.eta = self._eta(r.n, r.a.data_ptr(), r.nrhs, r.b.data_ptr(), r.x.data_ptr())
+                r.eta = self._eta(worker, r.n, r.a.data_ptr(), r.nrhs, r.b.data_ptr(), r.x.data_ptr())
 
-    def _run_single(self, r):
+    def _job_single(self, worker, job):
+        r, resident = job  # resident: (handle, kind, a) for a reuse request, else None
         lib, torch = self.lib, self.torch
         n = r.n
-        if r.reuse_id is not None:
-            if r.reuse_id not in self.resident:
-                r.status, r.done, r.path = 1, time.perf_counter(), "reuse"  # InvalidArg: not resident
-                return
-            self.resident.move_to_end(r.reuse_id)
-            h, kind, _rn, _w, a, _nb = self.resident[r.reuse_id]
+        sp = self._sptr(worker)
+        if resident is not None:
+            h, kind, a = resident
             solve = lib.rlg_lu_solve_f64_dev if kind == KIND_LU else lib.rlg_chol_solve_f64_dev
-            r.status = solve(h, r.nrhs, r.b.data_ptr(), n, r.x.data_ptr(), n, None)
-            assert lib.rlg_synchronize() == 0
+            r.status = solve(h, r.nrhs, r.b.data_ptr(), n, r.x.data_ptr(), n, sp)
+            worker.stream.synchronize()
             r.done, r.path = time.perf_counter(), "reuse"
             if r.check and r.status == 0:
-                r.eta = self._eta(n, a.data_ptr(), r.nrhs, r.b.data_ptr(), r.x.data_ptr())
+                r.eta = self._eta(worker, n, a.data_ptr(), r.nrhs, r.b.data_ptr(), r.x.data_ptr())
             return
         if self.mixed and r.keep_id is None and r.kind == KIND_LU:
-            opts = _lib.MixedOpts(1, 0, self.tol, 30, 1)
+            opts = _lib.MixedOpts(_lib.PREC_F16X3, 0, self.tol, 30, 1, _lib.PIVOT_PARTIAL)
             rep = _lib.MixedReport()
             r.status = lib.rlg_solve_mixed_f64_dev(n, r.a.data_ptr(), n, r.nrhs, r.b.data_ptr(), n,
-                                                   r.x.data_ptr(), n, C.byref(opts), C.byref(rep), None)
-            assert lib.rlg_synchronize() == 0
+                                                   r.x.data_ptr(), n, C.byref(opts), C.byref(rep), sp)
+            worker.stream.synchronize()
             r.done, r.path, r.eta, r.fell_back = time.perf_counter(), "single_mixed", rep.backward_error, rep.fell_back
             return
-        work = torch.empty(n * n, dtype=torch.float64, device=self.dev)
-        work.copy_(r.a)
-        torch.cuda.synchronize()
+        with torch.cuda.stream(worker.stream):
+            work = torch.empty(n * n, dtype=torch.float64, device=self.dev)
+        lib.rlg_dev_copy(work.data_ptr(), r.a.data_ptr(), 8 * n * n, sp)
         h = C.c_void_p()
         if r.kind == KIND_LU:
-            r.status = lib.rlg_lu_decompose_f64_dev(n, work.data_ptr(), n, None, C.byref(h))
+            r.status = lib.rlg_lu_decompose_f64_dev(n, work.data_ptr(), n, sp, C.byref(h))
             if r.status == 0:
-                r.status = lib.rlg_lu_solve_f64_dev(h, r.nrhs, r.b.data_ptr(), n, r.x.data_ptr(), n, None)
+                r.status = lib.rlg_lu_solve_f64_dev(h, r.nrhs, r.b.data_ptr(), n, r.x.data_ptr(), n, sp)
         else:
-            r.status = lib.rlg_chol_decompose_f64_dev(n, work.data_ptr(), n, None, C.byref(h))
+            r.status = lib.rlg_chol_decompose_f64_dev(n, work.data_ptr(), n, sp, C.byref(h))
             if r.status == 0:
-                r.status = lib.rlg_chol_solve_f64_dev(h, r.nrhs, r.b.data_ptr(), n, r.x.data_ptr(), n, None)
-        assert lib.rlg_synchronize() == 0
+                r.status = lib.rlg_chol_solve_f64_dev(h, r.nrhs, r.b.data_ptr(), n, r.x.data_ptr(), n, sp)
+        worker.stream.synchronize()
         r.done, r.path = time.perf_counter(), "single"
         if r.check and r.status == 0:
-            r.eta = self._eta(n, r.a.data_ptr(), r.nrhs, r.b.data_ptr(), r.x.data_ptr())
+            r.eta = self._eta(worker, n, r.a.data_ptr(), r.nrhs, r.b.data_ptr(), r.x.data_ptr())
         free = lib.rlg_lu_free if r.kind == KIND_LU else lib.rlg_chol_free
         if r.keep_id is not None and r.status == 0 and h.value:
-            if r.keep_id in self.resident:
-                self._drop(r.keep_id)
-            nbytes = 16 * n * n  # the factor and the original matrix
-            self.resident[r.keep_id] = (h, r.kind, n, work, r.a, nbytes)
-            self.resident_bytes += nbytes
+            r._kept = (h, work)  # handed to the dispatcher, which makes it resident
         elif h.value:
             free(h)
 
@@ -204,35 +256,119 @@ class ServingLoop:
         for r in requests:
             r.t_arr = t0 + r.arrival
         pending = collections.deque(sorted(requests, key=lambda r: r.t_arr))
-        while pending:
+        small_q = []          # admitted one-shot systems of order <= 512
+        single_q = collections.deque()  # admitted per-system requests
+        waiting_reuse = []    # reuse requests whose factor is still being computed
+        keeping = {}          # keep_id -> in-flight request
+        in_flight_bytes = 0
+        in_use = collections.Counter()  # resident id -> requests using it
+        outstanding = len(requests)
+        failure = None
+
+        def finish_now(r, status, path):
+            nonlocal outstanding
+            r.status, r.done, r.path = status, time.perf_counter(), path
+            outstanding -= 1
+
+        while outstanding:
+            # ---- completions (non-blocking)
+            while True:
+                try:
+                    worker, arg, err = self.completions.get_nowait()
+                except queue.Empty:
+                    break
+                worker.busy = False
+                if err is not None and failure is None:
+                    failure = err
+                done = arg if isinstance(arg, list) else [arg[0]]
+                for r in done:
+                    outstanding -= 1
+                    in_flight_bytes -= r.need
+                    if r.reuse_id is not None:
+                        in_use[r.reuse_id] -= 1
+                    if r.keep_id is not None:
+                        keeping.pop(r.keep_id, None)
+                        kept, r._kept = r._kept, None
+                        if kept is not None:
+                            if r.keep_id in self.resident:
+                                self._drop(r.keep_id)
+                            nbytes = 16 * r.n * r.n  # the factor and the original matrix
+                            self.resident[r.keep_id] = (kept[0], r.kind, r.n, kept[1], r.a, nbytes)
+                            self.resident_bytes += nbytes
+            if failure is not None:
+                raise failure
+            # ---- reuse requests whose factor has since been computed
+            still = []
+            for r in waiting_reuse:
+                if r.reuse_id in keeping:
+                    still.append(r)
+                elif r.reuse_id in self.resident:
+                    single_q.append(r)
+                else:
+                    finish_now(r, 1, "reuse")  # InvalidArg: the factor is not resident
+            waiting_reuse = still
+            # ---- admission in arrival order
             now = time.perf_counter()
-            if pending[0].t_arr > now:
-                time.sleep(min(pending[0].t_arr - now, 1e-3))
-                continue
-            admitted = []
-            used = 0
             while pending and pending[0].t_arr <= now:
                 r = pending[0]
-                need = self._need(r) + (16 * r.n * r.n if r.keep_id is not None else 0)
-                protect = {q.reuse_id for q in admitted if q.reuse_id is not None}
-                if r.reuse_id is not None:
-                    protect.add(r.reuse_id)
+                need = self._need(r)
                 if need > self.budget:
                     pending.popleft()
-                    r.status, r.done, r.path = RLG_E_OOM, time.perf_counter(), "rejected"
+                    finish_now(r, RLG_E_OOM, "rejected")
                     continue
-                if not self._evict_for(used + need, protect):
-                    break  # wait for this pass to finish
+                protect = set(k for k, c in in_use.items() if c > 0)
+                protect |= {q.reuse_id for q in list(single_q) + waiting_reuse if q.reuse_id is not None}
+                if r.reuse_id is not None:
+                    protect.add(r.reuse_id)
+                if in_flight_bytes + need > self.budget or not self._evict_for(in_flight_bytes + need, protect):
+                    break  # wait for work in flight to finish
                 pending.popleft()
-                admitted.append(r)
-                used += need
-            small = [r for r in admitted
-                     if r.n <= SMALL_MAX and r.keep_id is None and r.reuse_id is None]
-            singles = [r for r in admitted if r not in small]
-            if small:
-                self._run_batch(small)
-            for r in singles:
-                self._run_single(r)
+                r.need = need
+                in_flight_bytes += need
+                if r.reuse_id is not None:
+                    if r.reuse_id in keeping:
+                        waiting_reuse.append(r)
+                    elif r.reuse_id in self.resident:
+                        single_q.append(r)
+                    else:
+                        in_flight_bytes -= need
+                        finish_now(r, 1, "reuse")
+                elif r.n <= SMALL_MAX and r.keep_id is None:
+                    small_q.append(r)
+                else:
+                    if r.keep_id is not None:
+                        keeping[r.keep_id] = r
+                    single_q.append(r)
+            # ---- dispatch
+            if small_q and not self.batch_worker.busy:
+                self.batch_worker.busy = True
+                self.batch_worker.jobs.put((ServingLoop._job_batch.__get__(self), small_q))
+                small_q = []
+            for w in self.slot_workers:
+                if not single_q:
+                    break
+                if w.busy:
+                    continue
+                r = single_q.popleft()
+                resident = None
+                if r.reuse_id is not None:
+                    self.resident.move_to_end(r.reuse_id)
+                    h, kind, _rn, _w, a, _nb = self.resident[r.reuse_id]
+                    resident = (h, kind, a)
+                    in_use[r.reuse_id] += 1
+                w.busy = True
+                w.jobs.put((ServingLoop._job_single.__get__(self), (r, resident)))
+            # ---- wait for the next event: a completion or the next arrival
+            if outstanding:
+                timeout = 1e-3
+                if pending:
+                    timeout = max(0.0, min(timeout, pending[0].t_arr - time.perf_counter()))
+                try:
+                    item = self.completions.get(timeout=timeout) if timeout > 0 else None
+                except queue.Empty:
+                    item = None
+                if item is not None:
+                    self.completions.put(item)  # handled at the top of the loop
         return requests
 
 

@@ -72,13 +72,50 @@ def test_mix_against_oracle_and_alone(env):
             else:
                 xo = orc.chol_solve(orc.chol_decompose(a), b[i])
             assert np.abs(x[i] - xo).max() / np.abs(xo).max() <= max(1e-13, 16 * a.shape[0] * k * U)
-    # each request alone gives the same bits
+    # each request alone (the only request of a run) gives the same bits
+    solo = serving.ServingLoop(lib, torch, dev)
     for (kind, a, b, _), q in zip(specs, mix):
         alone = request(torch, dev, q.rid, kind, a, b)
-        serving.ServingLoop(lib, torch, dev).run([alone])
+        solo.run([alone])
         torch.cuda.synchronize()
         assert alone.status == 0 and alone.path == q.path
         assert torch.equal(alone.x, q.x)
+    solo.close()
+    loop.close()
+
+
+def test_requests_overlap_and_keep_their_results(env):
+    """Continuous admission: 40 requests arrive over 20 ms while earlier ones
+    are on the GPU (orders 600-1500 on four slot workers, small ones batched
+    meanwhile); every result equals the one its request gets alone, and the
+    run takes less time than the same requests one after another."""
+    lib, torch, dev = env
+    r = wl.rng(85)
+    specs = []
+    for i in range(40):
+        n = int(r.integers(600, 1500)) if i % 2 == 0 else int(r.integers(8, 300))
+        kind = serving.KIND_LU if i % 3 else serving.KIND_CHOL
+        a, b = system(r, kind, n, 1)
+        specs.append((kind, a, b, 0.0005 * i))
+    loop = serving.ServingLoop(lib, torch, dev, slots=4)
+    mix = [request(torch, dev, i, k, a, b, t) for i, (k, a, b, t) in enumerate(specs)]
+    torch.cuda.synchronize()
+    t0 = __import__("time").perf_counter()
+    loop.run(mix)
+    t_mix = __import__("time").perf_counter() - t0
+    solo = serving.ServingLoop(lib, torch, dev, slots=1)
+    t_solo = 0.0
+    for (kind, a, b, _), q in zip(specs, mix):
+        assert q.status == 0 and q.eta <= 1e-13, (q.rid, q.n, q.path)
+        alone = request(torch, dev, q.rid, kind, a, b)
+        torch.cuda.synchronize()
+        t0 = __import__("time").perf_counter()
+        solo.run([alone])
+        t_solo += __import__("time").perf_counter() - t0
+        assert torch.equal(alone.x, q.x), (q.rid, q.n, q.path)
+    print(f"40 requests: {t_mix * 1e3:.1f} ms in the mix, {t_solo * 1e3:.1f} ms one after another")
+    assert t_mix < t_solo
+    solo.close()
     loop.close()
 
 

@@ -4,6 +4,7 @@ otherwise), admission by the memory budget (RLG_E_OOM for a request that can
 never fit), least-recently-used eviction of resident factors, and reuse of a
 resident factor.  Device pointers are host pointers here; the stub records
 calls instead of computing."""
+import contextlib
 import ctypes as C
 import types
 
@@ -37,6 +38,10 @@ class StubLib:
         return 0
 
     def rlg_synchronize(self):
+        return 0
+
+    def rlg_dev_copy(self, dst, src, n, stream):
+        C.memmove(dst, src, n)
         return 0
 
     # solvers
@@ -79,13 +84,30 @@ class StubLib:
         return 0
 
 
+class HostStream:
+    """Stands in for torch.cuda.Stream: work on the host is always complete."""
+    cuda_stream = 0
+
+    def __init__(self, device=None):
+        pass
+
+    def synchronize(self):
+        pass
+
+
+@contextlib.contextmanager
+def host_stream_ctx(stream):
+    yield stream
+
+
 def torch_cpu():
     """torch with the device calls the loop makes mapped onto the CPU."""
     t = types.SimpleNamespace()
     t.cat = torch.cat
+    t.frombuffer = torch.frombuffer
     t.empty = lambda *a, **k: torch.empty(*a, **{kk: v for kk, v in k.items() if kk != "device"})
     t.uint8, t.int64, t.float64 = torch.uint8, torch.int64, torch.float64
-    t.cuda = types.SimpleNamespace(synchronize=lambda: None)
+    t.cuda = types.SimpleNamespace(synchronize=lambda: None, Stream=HostStream, stream=host_stream_ctx)
     return t
 
 
@@ -97,7 +119,7 @@ def req(rid, kind, n, arrival=0.0, **kw):
 
 def test_routing_batch_and_single():
     lib = StubLib()
-    loop = serving.ServingLoop(lib, torch_cpu(), None)
+    loop = serving.ServingLoop(lib, torch_cpu(), "cpu")
     reqs = [req(0, serving.KIND_LU, 100), req(1, serving.KIND_CHOL, 512), req(2, serving.KIND_LU, 513),
             req(3, serving.KIND_CHOL, 900)]
     loop.run(reqs)
@@ -113,7 +135,7 @@ def test_routing_batch_and_single():
 
 def test_budget_rejects_what_never_fits():
     lib = StubLib()
-    loop = serving.ServingLoop(lib, torch_cpu(), None, mem_budget_bytes=1 << 20)
+    loop = serving.ServingLoop(lib, torch_cpu(), "cpu", mem_budget_bytes=1 << 20)
     big, small = req(0, serving.KIND_LU, 400), req(1, serving.KIND_LU, 100, arrival=0.0)
     loop.run([big, small])
     assert big.status == serving.RLG_E_OOM and big.path == "rejected"
@@ -124,7 +146,7 @@ def test_resident_reuse_and_lru_eviction():
     lib = StubLib()
     # room for two resident 128-order factors (16 n^2 bytes each) plus a pass
     budget = 2 * 16 * 128 * 128 + 8 * 128 * 129
-    loop = serving.ServingLoop(lib, torch_cpu(), None, mem_budget_bytes=budget)
+    loop = serving.ServingLoop(lib, torch_cpu(), "cpu", mem_budget_bytes=budget)
     k1 = req(0, serving.KIND_LU, 128, keep_id=1)
     k2 = req(1, serving.KIND_LU, 128, arrival=0.001, keep_id=2)
     loop.run([k1, k2])
@@ -144,7 +166,7 @@ def test_resident_reuse_and_lru_eviction():
 
 def test_mixed_mode_routes_large_lu_to_the_mixed_solve():
     lib = StubLib()
-    loop = serving.ServingLoop(lib, torch_cpu(), None, mixed=True)
+    loop = serving.ServingLoop(lib, torch_cpu(), "cpu", mixed=True)
     reqs = [req(0, serving.KIND_LU, 600), req(1, serving.KIND_CHOL, 600), req(2, serving.KIND_LU, 64)]
     loop.run(reqs)
     assert [r.path for r in reqs] == ["single_mixed", "single", "batched_mixed"]
