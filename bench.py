@@ -301,8 +301,8 @@ def main():
     if not args.no_extras:
         variants = {}
 
-        def mixed_once(a, b, x, nr, prec):
-            o = _lib.MixedOpts(PREC[prec], 0, 1e-13, 30, 1, PIVOT[args.pivot])
+        def mixed_once(a, b, x, nr, prec, refine=_lib.REFINE_IR):
+            o = _lib.MixedOpts(PREC[prec], refine, 1e-13, 30, 1, PIVOT[args.pivot])
             r = _lib.MixedReport()
             assert lib.rlg_synchronize() == 0
             t0 = time.perf_counter()
@@ -315,7 +315,8 @@ def main():
                                                 n, C.byref(eta), sptr)
             assert st == 0, _lib.last_error()
             return {"ms": round(ms, 3), "backward_error": eta.value, "ir_iterations": r.iterations,
-                    "fell_back": r.fell_back, "factor_prec": prec}
+                    "gmres_iterations": r.inner_iterations, "fell_back": r.fell_back, "factor_prec": prec,
+                    "refine": "gmres" if refine == _lib.REFINE_GMRES else "ir"}
 
         # 32 right-hand sides at kappa 1e4, both factor precisions
         g.manual_seed(23)
@@ -325,6 +326,7 @@ def main():
         for prec in ("f16x3", "f32", "f16"):
             mixed_once(A, B32, X32, 32, prec)
             variants[f"mixed_{prec}_32rhs_kappa1e4"] = mixed_once(A, B32, X32, 32, prec)
+        variants["mixed_f16_gmres_32rhs_kappa1e4"] = mixed_once(A, B32, X32, 32, "f16", _lib.REFINE_GMRES)
         mixed_once(A, B, X, nrhs, "f32")
         variants["mixed_f32_1rhs_kappa1e4"] = mixed_once(A, B, X, nrhs, "f32")
         # plain FP64 LU + 16 right-hand sides (factor and solve)
