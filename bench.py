@@ -65,6 +65,13 @@ def parse():
                    help="slot workers (streams) of the serving loop for systems above order 512")
     p.add_argument("--serving-only", action="store_true",
                    help="only run the serving traces (2,000 and 200 requests/s) and exit")
+    p.add_argument("--cpu-table", action="store_true",
+                   help="time the CPU restatement of the reference's path on one pinned core "
+                        "(GEMM 256, LU 256 + solve, LU and Cholesky fits extrapolated to 16384, "
+                        "a batched subsample) and print one JSON line; no GPU is used")
+    p.add_argument("--dgemm-only", action="store_true", help="only DGEMM 8192^3 (for profiling)")
+    p.add_argument("--batched-only", action="store_true",
+                   help="only the 20,000-system batched solve (for profiling)")
     p.add_argument("--cholesky-only", action="store_true",
                    help="only time Cholesky::decompose at n=16384 (two factorizations) and exit")
     return p.parse_args()
@@ -97,6 +104,9 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    if args.cpu_table:
+        print(json.dumps(cpu_table()), flush=True)
+        return
     import torch
 
     from rulinalg_amd import _lib
@@ -105,6 +115,12 @@ def main():
     dev = torch.device("cuda:0")
     if args.cholesky_only:
         print(json.dumps(cholesky_16384(lib, torch, dev, None)), flush=True)
+        return
+    if args.dgemm_only:
+        print(json.dumps(dgemm_8192(lib, torch, dev, None)), flush=True)
+        return
+    if args.batched_only:
+        print(json.dumps(batched(lib, torch, dev)), flush=True)
         return
     if args.serving_only:
         print(json.dumps({"offered_2000_per_s": serving_trace(lib, torch, dev, 2000, 1.0, 50, args.serve_slots),
@@ -548,6 +564,9 @@ def batched(lib, torch, dev, count=20000, nmin=8, nmax=512, seed=30):
     return out
 
 
+_SERVED = False  # set once a serving loop has run in this process
+
+
 def serving_trace(lib, torch, dev, rate, seconds, seed, slots=4):
     """Serving workload: Poisson arrivals at `rate` requests/s for `seconds`;
     n log-uniform on [16, 4096]; 70 % LU (Gaussian) / 30 % Cholesky
@@ -586,6 +605,8 @@ def serving_trace(lib, torch, dev, rate, seconds, seed, slots=4):
         reqs = [serving.Request(i, sp["kind"], sp["n"], 1, sp["a"], sp["b"], torch.empty_like(sp["b"]),
                                 float(arrivals[i]), keep_id=sp.get("keep"), reuse_id=sp.get("reuse"))
                 for i, sp in enumerate(specs)]
+        global _SERVED
+        _SERVED = True
         loop = serving.ServingLoop(lib, torch, dev, mem_budget_bytes=64 << 30, mixed=mixed, slots=slots)
         loop.run(reqs)
         loop.close()
@@ -601,8 +622,15 @@ def serving_trace(lib, torch, dev, rate, seconds, seed, slots=4):
 
 def extras(lib, torch, dev, sptr):
     """DGEMM 8192^3 and Cholesky 16384 (plain FP64) timings."""
+    out = {"dgemm_8192": dgemm_8192(lib, torch, dev, sptr)}
+    out["cholesky_16384"] = cholesky_16384(lib, torch, dev, sptr)
+    return out
+
+
+def dgemm_8192(lib, torch, dev, sptr):
+    """C = A B at 8192^3 in FP64 (U[-1, 1] operands), best of three after a
+    warm-up, checked by a Freivalds probe."""
     from rulinalg_amd import _lib
-    out = {}
     g = torch.Generator(device=dev)
     m = 8192
     g.manual_seed(10)
@@ -619,16 +647,16 @@ def extras(lib, torch, dev, sptr):
         assert lib.rlg_synchronize() == 0
         if it:
             times.append((time.perf_counter() - t0) * 1e3)
-    # Freivalds probe: ||C z - A (B z)|| relative to ||A|| ||B||
+    # Freivalds probe: ||C z - A (B z)||inf / (||A||inf ||B||inf ||z||inf)
     z = torch.randn(m, 4, dtype=torch.float64, device=dev)
-    err = (c @ z - a @ (b @ z)).abs().max().item() / (a.abs().sum(1).max().item() * b.abs().max().item())
+    err = (c @ z - a @ (b @ z)).abs().max().item() / (
+        a.abs().sum(1).max().item() * b.abs().sum(1).max().item() * z.abs().max().item())
     tf = 2 * m ** 3 / (min(times) / 1e3) / 1e12
-    out["dgemm_8192"] = {"ms": round(min(times), 3), "tflops": round(tf, 2),
-                         "frac_fp64_peak": round(tf / 78.6, 4), "freivalds_rel": err}
+    assert err <= 1e-13, f"DGEMM 8192 Freivalds probe {err:.3e}"
     log(f"DGEMM 8192: {min(times):.2f} ms, {tf:.1f} TF/s")
     del a, b, c
-    out["cholesky_16384"] = cholesky_16384(lib, torch, dev, sptr)
-    return out
+    return {"ms": round(min(times), 3), "tflops": round(tf, 2), "frac_fp64_peak": round(tf / 78.6, 4),
+            "freivalds_rel": err}
 
 
 def cholesky_16384(lib, torch, dev, sptr):
@@ -743,14 +771,89 @@ def cpu_baseline(cpu_n, n):
             "host": cpu_model(), "pinned_core": pin.core, "nproc": os.cpu_count()}
 
 
+def cpu_table():
+    """The CPU rows reported beside the GPU numbers: rulinalg's algorithms as
+    restated by the oracle (bit-exact for LU, Cholesky and the
+    substitutions; a blocked single-thread DGEMM standing in for
+    matrixmultiply::dgemm), one pinned core.  Small orders are the median of
+    five runs; LU and Cholesky at 16384 are c n^3 fits over orders 512-2048,
+    extrapolated (the fit's largest measured order is also reported)."""
+    from oracle import oracle as orc
+    from rulinalg_amd import workloads as wl
+
+    def median_s(fn, reps=5):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    out = {"host": cpu_model(), "nproc": os.cpu_count(), "cores": 1, "kind": "port"}
+    with one_core() as pin:
+        out["pinned_core"] = pin.core
+        ga, gb = wl.uniform(256, 256, 0), wl.uniform(256, 256, 1)
+        t = median_s(lambda: orc.gemm_blocked(ga, gb))
+        out["gemm_256_f64"] = {"s": t, "gflops": round(2 * 256 ** 3 / t / 1e9, 3)}
+        a = np.ascontiguousarray(wl.conditioned(256, 1e4, 2, 3))
+        b = wl.gaussian(256, 1, 4)[:, 0]
+
+        def lu_and_solve(a=a, b=b):
+            lu, perm, _ = orc.lu_decompose(a)
+            orc.lu_solve(lu, perm, b)
+
+        t = median_s(lu_and_solve)
+        out["lu_256_solve"] = {"ms": round(t * 1e3, 3)}
+        fits = {}
+        for name in ("lu", "cholesky"):
+            pts = []
+            for nn in (512, 1024, 2048):
+                if name == "lu":
+                    m = np.ascontiguousarray(wl.conditioned(nn, 1e4, 20 + nn, 21 + nn))
+                    t = median_s(lambda m=m: orc.lu_decompose(m), reps=1 if nn >= 2048 else 3)
+                else:
+                    m = np.ascontiguousarray(wl.spd(nn, 30 + nn))
+                    t = median_s(lambda m=m: orc.chol_decompose(m), reps=1 if nn >= 2048 else 3)
+                pts.append((nn, t))
+            # least-squares c in t = c n^3
+            c = sum(t * nn ** 3 for nn, t in pts) / sum(float(nn) ** 6 for nn, _ in pts)
+            fits[name] = {"measured": [{"n": nn, "s": round(t, 4)} for nn, t in pts],
+                          "c": c, "extrapolated_s_at_16384": round(c * 16384.0 ** 3, 1)}
+        out["fits"] = fits
+        sizes = np.random.default_rng(5).integers(8, 513, 500)
+        systems = []
+        for i, nn in enumerate(sizes):
+            if i % 2 == 0:
+                systems.append(("lu", np.ascontiguousarray(wl.conditioned(int(nn), 1e2, 40 + i, 41 + i)),
+                                wl.gaussian(int(nn), 1, 42 + i)[:, 0]))
+            else:
+                systems.append(("chol", np.ascontiguousarray(wl.spd(int(nn), 50 + i)),
+                                wl.gaussian(int(nn), 1, 52 + i)[:, 0]))
+        t0 = time.perf_counter()
+        for kind, m, rhs in systems:
+            if kind == "lu":
+                lu, perm, _ = orc.lu_decompose(m)
+                orc.lu_solve(lu, perm, rhs)
+            else:
+                orc.chol_solve(orc.chol_decompose(m), rhs)
+        dt = time.perf_counter() - t0
+        out["batched_subsample"] = {"systems": len(systems), "s": round(dt, 3),
+                                    "solves_per_s": round(len(systems) / dt, 1),
+                                    "what": "n uniform in [8, 512], alternating LU (kappa 1e2) and "
+                                            "Cholesky, one right-hand side each"}
+    return out
+
+
 if __name__ == "__main__":
     main()
-    # The result line is printed and flushed.  The process then ends without
-    # running the interpreter's finalizers: after the serving loop's worker
-    # threads have driven their own HIP streams, the teardown of the HIP
-    # runtime and PyTorch once ended in a segmentation fault after every
-    # result had been printed, which would turn a complete run into a failed
-    # exit status.
-    sys.stdout.flush()
-    sys.stderr.flush()
-    os._exit(0)
+    if _SERVED:
+        # The result line is printed and flushed.  After the serving loop's
+        # worker threads have driven their own HIP streams, the process ends
+        # without running the interpreter's finalizers: the teardown of the
+        # HIP runtime and PyTorch once ended in a segmentation fault after
+        # every result had been printed, which would turn a complete run into
+        # a failed exit status.  Runs without the serving loop end normally,
+        # so that a profiler's own exit handlers write their results.
+        sys.stdout.flush()
+        sys.stderr.flush()
+        os._exit(0)
