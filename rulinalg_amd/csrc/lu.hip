@@ -184,42 +184,50 @@ __device__ __forceinline__ bool xwait(unsigned &spins, int *timeout) {
 template <class T> struct TilePad { static constexpr int value = 16 / sizeof(T); };
 template <class T> constexpr int TW = PNB + TilePad<T>::value;
 
-// a[r][c] = a[r][c] - mult * pm[c] over the columns from C0 on, as one
-// straight sweep.  pm is the pivot row with zeros in the columns that must
-// stay as they are (x - mult * 0 = x, up to the sign of a zero x), so every
-// column is read and written without a branch or select: a per-column test
-// gets compiled into a conditional store, one LDS round trip per column.
-// Columns w..63 of the tile and of pm hold zeros and stay zero.
-template <class T, int C0>
+// a[r][c] = a[r][c] - mult * pm[c] for the columns c > j from chunk C0 on,
+// as one straight sweep: the row part is loaded, then stored.  The first
+// 8-column chunk may hold columns <= j.  With EXACT they keep their value,
+// by a select on the computed value, as in the reference, which never
+// touches them (x - mult * 0 would turn a -0 into +0, and a non-finite
+// multiplier would reach them).  Without EXACT they take x - mult * 0 (pm is
+// zero there), which differs from x only in the sign of a zero: the mixed
+// factorization, whose factors only precondition the FP64 refinement, uses
+// that form, which needs fewer registers.  Every column is stored
+// unconditionally: a per-column test around the store gets compiled into a
+// conditional store, one LDS round trip per column.  Columns w..63 of the
+// tile and of pm hold zeros and stay zero.
+template <class T, int C0, bool EXACT>
 __device__ __forceinline__ void elim_row(T (*__restrict__ tile)[TW<T>], const T *__restrict__ pm,
-                                         int r, T mult) {
+                                         int r, int j, T mult) {
 #pragma clang fp contract(off)
     T x[PNB - C0];
 #pragma unroll
     for (int c = C0; c < PNB; ++c) x[c - C0] = tile[r][c];
 #pragma unroll
-    for (int c = C0; c < PNB; ++c) tile[r][c] = x[c - C0] - mult * pm[c];
+    for (int c = C0; c < PNB; ++c) {
+        const T v = x[c - C0] - mult * pm[c];
+        tile[r][c] = (!EXACT || c >= C0 + 8 || c > j) ? v : x[c - C0];
+    }
 }
 
-// elim_row from the 8-column chunk that holds column j + 1 (pm is zero at
-// columns <= j + 1 for the pending update of column j's step).
-template <class T>
+// elim_row from the 8-column chunk that holds column j + 1.
+template <class T, bool EXACT>
 __device__ __forceinline__ void elim_row_from(T (*__restrict__ tile)[TW<T>], const T *__restrict__ pm,
                                               int r, int j, T mult) {
     switch ((j + 1) >> 3) {
-        case 0: elim_row<T, 0>(tile, pm, r, mult); break;
-        case 1: elim_row<T, 8>(tile, pm, r, mult); break;
-        case 2: elim_row<T, 16>(tile, pm, r, mult); break;
-        case 3: elim_row<T, 24>(tile, pm, r, mult); break;
-        case 4: elim_row<T, 32>(tile, pm, r, mult); break;
-        case 5: elim_row<T, 40>(tile, pm, r, mult); break;
-        case 6: elim_row<T, 48>(tile, pm, r, mult); break;
-        case 7: elim_row<T, 56>(tile, pm, r, mult); break;
+        case 0: elim_row<T, 0, EXACT>(tile, pm, r, j, mult); break;
+        case 1: elim_row<T, 8, EXACT>(tile, pm, r, j, mult); break;
+        case 2: elim_row<T, 16, EXACT>(tile, pm, r, j, mult); break;
+        case 3: elim_row<T, 24, EXACT>(tile, pm, r, j, mult); break;
+        case 4: elim_row<T, 32, EXACT>(tile, pm, r, j, mult); break;
+        case 5: elim_row<T, 40, EXACT>(tile, pm, r, j, mult); break;
+        case 6: elim_row<T, 48, EXACT>(tile, pm, r, j, mult); break;
+        case 7: elim_row<T, 56, EXACT>(tile, pm, r, j, mult); break;
         default: break;  // j = 63: nothing right of column j
     }
 }
 
-template <class T>
+template <class T, bool EXACT>
 __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long lda, int k, int m,
                                                        int w, int64_t *__restrict__ ipiv,
                                                        int64_t *__restrict__ info, int n,
@@ -284,15 +292,18 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
             bi = wl >= 0 ? lane_get(bi, wl) : INT_MAX;
         }
         {   // the wave's candidate row and row j, brought up to date with pivot
-            // j - 1 (the arithmetic step C applies to the rows in place)
+            // j - 1 right of column j (the arithmetic step C applies to the
+            // rows in place); columns <= j are final and stay as they are
 #pragma clang fp contract(off)
             if (bi != INT_MAX) {
                 const T x = tile[bi - r0][lane];
-                wrow[wave][lane] = j > 0 ? x - tile[bi - r0][j - 1] * pmprev[lane] : x;
+                const T v = x - tile[bi - r0][j > 0 ? j - 1 : 0] * pmprev[lane];
+                wrow[wave][lane] = (j > 0 && lane > j) ? v : x;
             }
             if (own_j && (((j - r0) % PTHREADS) >> 6) == wave) {
                 const T x = tile[j - r0][lane];
-                jrow[lane] = j > 0 ? x - tile[j - r0][j - 1] * pmprev[lane] : x;
+                const T v = x - tile[j - r0][j > 0 ? j - 1 : 0] * pmprev[lane];
+                jrow[lane] = (j > 0 && lane > j) ? v : x;
             }
         }
         if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
@@ -409,21 +420,21 @@ __global__ __launch_bounds__(PTHREADS) void k_lu_panel(T *__restrict__ A, long l
 #pragma unroll
                 for (int h = 0; h < RPT; ++h) {
                     const int r = tid + h * PTHREADS;
-                    if (r < rows && r0 + r >= j) elim_row_from(tile, pmprev, r, j, tile[r][j - 1]);
+                    if (r < rows && r0 + r >= j) elim_row_from<T, EXACT>(tile, pmprev, r, j, tile[r][j - 1]);
                 }
             } else if (wave > 0) {
                 // own rows, then wave 0's 64 RPT rows spread over the 192 threads
 #pragma unroll
                 for (int h = 0; h < RPT; ++h) {
                     const int r = tid + h * PTHREADS;
-                    if (r < rows && r0 + r >= j) elim_row_from(tile, pmprev, r, j, tile[r][j - 1]);
+                    if (r < rows && r0 + r >= j) elim_row_from<T, EXACT>(tile, pmprev, r, j, tile[r][j - 1]);
                 }
                 constexpr int XTRA = (64 * RPT + PTHREADS - 64 - 1) / (PTHREADS - 64);
 #pragma unroll
                 for (int q = 0; q < XTRA; ++q) {
                     const int e = (tid - 64) + q * (PTHREADS - 64);
                     const int r = (e % 64) + (e / 64) * PTHREADS;
-                    if (e < 64 * RPT && r < rows && r0 + r >= j) elim_row_from(tile, pmprev, r, j, tile[r][j - 1]);
+                    if (e < 64 * RPT && r < rows && r0 + r >= j) elim_row_from<T, EXACT>(tile, pmprev, r, j, tile[r][j - 1]);
                 }
             }
         }
@@ -745,6 +756,16 @@ __global__ __launch_bounds__(256) void k_trsm_llu(T *__restrict__ A, long lda, i
 // Set by lu_factor_lowp for its duration: FP32 panels are then factored with
 // tournament pivoting (tslu.hip) in this workspace.
 static thread_local void *g_tslu_work = nullptr;
+// Cleared by lu_factor_lowp for its duration: the mixed factorization's
+// panels may turn a -0 left of the pivot column into +0 (see elim_row); the
+// drop-in factorizations keep the reference's zeros exactly.
+static thread_local bool g_panel_exact = true;
+
+template <class T>
+static const void *panel_kernel() {
+    return g_panel_exact ? reinterpret_cast<const void *>(k_lu_panel<T, true>)
+                         : reinterpret_cast<const void *>(k_lu_panel<T, false>);
+}
 
 template <class T>
 static int panel(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, int64_t *ipiv,
@@ -776,11 +797,10 @@ template <class T>
 static int panel_rows_max(size_t *out) {
     Device *d = nullptr;
     RLG_TRY(current_device(&d));
-    static thread_local int per_cu[2] = {-1, -1};
-    int &bpc = per_cu[sizeof(T) == 8];
+    static thread_local int per_cu[2][2] = {{-1, -1}, {-1, -1}};
+    int &bpc = per_cu[sizeof(T) == 8][g_panel_exact];
     if (bpc < 0)
-        RLG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &bpc, reinterpret_cast<const void *>(k_lu_panel<T>), PTHREADS, 0));
+        RLG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, panel_kernel<T>(), PTHREADS, 0));
     size_t g = (size_t)bpc * (size_t)d->num_cus;
     if (g > (size_t)GMAX) g = GMAX;
     *out = g * PanelRows<T>::value;
@@ -808,8 +828,7 @@ static int panel(hipStream_t s, T *a, size_t lda, size_t k, size_t m, size_t w, 
     xword *xch_ = lw.xch;
     int *to_ = lw.timeout;
     void *args[] = {&a_, &lda_, &k_, &m_, &w_, &ipiv, &info, &n_, &xch_, &to_};
-    RLG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_lu_panel<T>), dim3(G),
-                                       dim3(PTHREADS), args, 0, s));
+    RLG_HIP(hipLaunchCooperativeKernel(panel_kernel<T>(), dim3(G), dim3(PTHREADS), args, 0, s));
     return RLG_OK;
 }
 
@@ -1001,6 +1020,7 @@ int lu_factor_lowp(hipStream_t s, int prec, int pivot, size_t n, float *a, size_
     g_lowp_pack = w + align_up(lu_work_bytes(n, 4));
     g_lowp_pack_bytes = gemm_lowp_pack_bytes(n, n, ONB);
     g_tslu_work = nullptr;
+    g_panel_exact = false;
     int st = RLG_OK;
     if (pivot == RLG_PIVOT_TOURNAMENT) {
         g_tslu_work = w + align_up(lu_work_bytes(n, 4)) + align_up(g_lowp_pack_bytes);
@@ -1012,6 +1032,7 @@ int lu_factor_lowp(hipStream_t s, int prec, int pivot, size_t n, float *a, size_
     g_lowp_pack = nullptr;
     g_lowp_pack_bytes = 0;
     g_tslu_work = nullptr;
+    g_panel_exact = true;
     return st;
 }
 

@@ -260,6 +260,27 @@ def test_lu_bit_exact_single_panel(n):
     assert np.array_equal(lu.packed(), lu_or)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [33, 48, 64])
+def test_lu_single_panel_bits_with_zeros(n, dtype):
+    """Small-integer matrices (exact ties and many exact zeros, so zero
+    multipliers of either sign): inside one panel the packed factor equals
+    the oracle's bit for bit, signs of zeros included.  The reference's
+    elimination touches only the columns right of the pivot column
+    (lu.rs:603-616), so a -0 multiplier stays -0; np.array_equal alone
+    would accept +0 for it."""
+    a = np.random.Generator(np.random.Philox(4000 + n)).integers(-3, 4, (n, n)).astype(dtype)
+    lu_or, perm_or, ipiv_or = orc.lu_decompose(a, dtype=dtype)
+    lu = rl.PartialPivLu.decompose(rl.Matrix(a))
+    assert np.array_equal(lu.ipiv(), ipiv_or)
+    got = np.ascontiguousarray(lu.packed(), dtype=dtype)
+    want = np.ascontiguousarray(lu_or, dtype=dtype)
+    bits = np.uint64 if dtype == np.float64 else np.uint32
+    assert ((want == 0) & np.signbit(want)).any(), "the case must hold a -0"
+    assert np.array_equal(got.view(bits), want.view(bits)), \
+        f"{int((got.view(bits) != want.view(bits)).sum())} elements differ in their bits"
+
+
 @pytest.mark.parametrize("n", [65, 127, 128, 129, 200, 255, 256, 257, 300, 513, 777, 2304])
 def test_lu_vs_oracle(n):
     a = wl.gaussian(n, n, 2000 + n)
