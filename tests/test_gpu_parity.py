@@ -451,6 +451,21 @@ def test_chol_bit_exact_single_block(n):
     assert np.array_equal(l, orc.chol_unpack(w_or))
 
 
+@pytest.mark.parametrize("n", [33, 64])
+def test_chol_single_block_bits_banded(n):
+    """A pentadiagonal integer SPD matrix (4 on the diagonal, -1 on the two
+    sub- and superdiagonals each side): its factor holds exact zeros outside
+    the band; inside one block the factor equals the oracle's bit for bit,
+    signs of zeros included (np.array_equal alone would accept either)."""
+    a = 6.0 * np.eye(n)
+    for d in (1, 2):
+        a -= np.eye(n, k=d) + np.eye(n, k=-d)
+    l_or = orc.chol_unpack(orc.chol_decompose(a))
+    l = rl.Cholesky.decompose(rl.Matrix(a)).unpack().to_numpy()
+    assert (l_or == 0).sum() > n * (n - 5) // 2, "the factor must hold exact zeros"
+    assert np.array_equal(l.view(np.uint64), l_or.view(np.uint64))
+
+
 @pytest.mark.parametrize("n", [65, 128, 200, 256, 257, 520])
 def test_chol_vs_oracle(n):
     a = wl.spd(n, 600 + n)
