@@ -61,6 +61,8 @@ def parse():
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run three device copies of MIB MiB (the plain copy kernel that "
                         "calibrates the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    p.add_argument("--serve-pivot", default="tournament", choices=sorted(PIVOT),
+                   help="pivoting of the serving loop's mixed large-system factorizations")
     p.add_argument("--serve-slots", type=int, default=4,
                    help="slot workers (streams) of the serving loop for systems above order 512")
     p.add_argument("--serving-only", action="store_true",
@@ -123,8 +125,9 @@ def main():
         print(json.dumps(batched(lib, torch, dev)), flush=True)
         return
     if args.serving_only:
-        print(json.dumps({"offered_2000_per_s": serving_trace(lib, torch, dev, 2000, 1.0, 50, args.serve_slots),
-                          "offered_200_per_s": serving_trace(lib, torch, dev, 200, 2.0, 51, args.serve_slots)}),
+        piv = PIVOT[args.serve_pivot]
+        print(json.dumps({"offered_2000_per_s": serving_trace(lib, torch, dev, 2000, 1.0, 50, args.serve_slots, piv),
+                          "offered_200_per_s": serving_trace(lib, torch, dev, 200, 2.0, 51, args.serve_slots, piv)}),
               flush=True)
         return
     # torch prepares the inputs on its own stream and is synchronized before
@@ -385,8 +388,9 @@ def main():
         result.update(extras(lib, torch, dev, sptr))
         result["batched"] = batched(lib, torch, dev)
         # the serving loop at the configured rate and at a rate one GPU sustains
-        result["serving"] = {"offered_2000_per_s": serving_trace(lib, torch, dev, 2000, 1.0, 50, args.serve_slots),
-                             "offered_200_per_s": serving_trace(lib, torch, dev, 200, 2.0, 51, args.serve_slots)}
+        piv = PIVOT[args.serve_pivot]
+        result["serving"] = {"offered_2000_per_s": serving_trace(lib, torch, dev, 2000, 1.0, 50, args.serve_slots, piv),
+                             "offered_200_per_s": serving_trace(lib, torch, dev, 200, 2.0, 51, args.serve_slots, piv)}
 
     # ------------------------------------------------------------ CPU baseline
     if not args.no_cpu:
@@ -567,7 +571,7 @@ def batched(lib, torch, dev, count=20000, nmin=8, nmax=512, seed=30):
 _SERVED = False  # set once a serving loop has run in this process
 
 
-def serving_trace(lib, torch, dev, rate, seconds, seed, slots=4):
+def serving_trace(lib, torch, dev, rate, seconds, seed, slots=4, mixed_pivot=1):
     """Serving workload: Poisson arrivals at `rate` requests/s for `seconds`;
     n log-uniform on [16, 4096]; 70 % LU (Gaussian) / 30 % Cholesky
     (G G^T / n + I); one right-hand side; about 10 % of the requests keep
@@ -607,7 +611,8 @@ def serving_trace(lib, torch, dev, rate, seconds, seed, slots=4):
                 for i, sp in enumerate(specs)]
         global _SERVED
         _SERVED = True
-        loop = serving.ServingLoop(lib, torch, dev, mem_budget_bytes=64 << 30, mixed=mixed, slots=slots)
+        loop = serving.ServingLoop(lib, torch, dev, mem_budget_bytes=64 << 30, mixed=mixed, slots=slots,
+                                   mixed_pivot=mixed_pivot)
         loop.run(reqs)
         loop.close()
         del loop

@@ -17,7 +17,15 @@ Admitted requests go to workers, each a host thread with its own HIP stream:
   the first free slot worker, which runs the per-system path
   (``rlg_{lu,chol}_decompose_f64_dev`` + ``_solve_f64_dev``, or
   ``rlg_solve_mixed_f64_dev`` for one-shot mixed LU) on its stream; the slot
-  workers' systems run on the device at the same time.
+  workers' systems run on the device at the same time.  In mixed mode the
+  large systems' low-precision factorizations use tournament pivoting
+  (``mixed_pivot``, default ``PIVOT_TOURNAMENT``): its panels combine their
+  workgroups' candidates without waiting for one another, so they are
+  ordinary launches that run beside the other slots' work, where the
+  partial-pivoting panel is a cooperative launch that the runtime does not
+  overlap with other streams' kernels.  The FP64 refinement and fallback
+  make the pivot choice invisible in the result's accuracy.  The plain mode
+  keeps the reference's partial pivoting.
 
 Every call a worker makes is ordered on its own stream and a worker waits
 only for its own stream, so no request waits for another's work to finish;
@@ -98,10 +106,12 @@ class _Worker(threading.Thread):
 
 
 class ServingLoop:
-    def __init__(self, lib, torch, dev, mem_budget_bytes=64 << 30, mixed=False, tol=1e-13, slots=4):
+    def __init__(self, lib, torch, dev, mem_budget_bytes=64 << 30, mixed=False, tol=1e-13, slots=4,
+                 mixed_pivot=_lib.PIVOT_TOURNAMENT):
         self.lib, self.torch, self.dev = lib, torch, dev
         self.budget = mem_budget_bytes
         self.mixed = mixed
+        self.mixed_pivot = mixed_pivot
         self.tol = tol
         self.resident = collections.OrderedDict()  # id -> (handle, kind, n, work, a, bytes)
         self.resident_bytes = 0
@@ -219,7 +229,7 @@ class ServingLoop:
                 r.eta = self._eta(worker, n, a.data_ptr(), r.nrhs, r.b.data_ptr(), r.x.data_ptr())
             return
         if self.mixed and r.keep_id is None and r.kind == KIND_LU:
-            opts = _lib.MixedOpts(_lib.PREC_F16X3, 0, self.tol, 30, 1, _lib.PIVOT_PARTIAL)
+            opts = _lib.MixedOpts(_lib.PREC_F16X3, 0, self.tol, 30, 1, self.mixed_pivot)
             rep = _lib.MixedReport()
             r.status = lib.rlg_solve_mixed_f64_dev(n, r.a.data_ptr(), n, r.nrhs, r.b.data_ptr(), n,
                                                    r.x.data_ptr(), n, C.byref(opts), C.byref(rep), sp)

@@ -84,11 +84,15 @@ def test_mix_against_oracle_and_alone(env):
     loop.close()
 
 
-def test_requests_overlap_and_keep_their_results(env):
+@pytest.mark.parametrize("mixed", [False, True], ids=["plain", "mixed"])
+def test_requests_overlap_and_keep_their_results(env, mixed):
     """Continuous admission: 40 requests arrive over 20 ms while earlier ones
     are on the GPU (orders 600-1500 on four slot workers, small ones batched
     meanwhile); every result equals the one its request gets alone, and the
-    run takes less time than the same requests one after another."""
+    run takes less time than the same requests one after another.  In mixed
+    mode the large LU systems are factored with tournament pivoting (the
+    loop's default), whose result does not depend on which workgroup merges
+    a tree node."""
     lib, torch, dev = env
     r = wl.rng(85)
     specs = []
@@ -97,13 +101,13 @@ def test_requests_overlap_and_keep_their_results(env):
         kind = serving.KIND_LU if i % 3 else serving.KIND_CHOL
         a, b = system(r, kind, n, 1)
         specs.append((kind, a, b, 0.0005 * i))
-    loop = serving.ServingLoop(lib, torch, dev, slots=4)
+    loop = serving.ServingLoop(lib, torch, dev, slots=4, mixed=mixed)
     mix = [request(torch, dev, i, k, a, b, t) for i, (k, a, b, t) in enumerate(specs)]
     torch.cuda.synchronize()
     t0 = __import__("time").perf_counter()
     loop.run(mix)
     t_mix = __import__("time").perf_counter() - t0
-    solo = serving.ServingLoop(lib, torch, dev, slots=1)
+    solo = serving.ServingLoop(lib, torch, dev, slots=1, mixed=mixed)
     t_solo = 0.0
     for (kind, a, b, _), q in zip(specs, mix):
         assert q.status == 0 and q.eta <= 1e-13, (q.rid, q.n, q.path)
