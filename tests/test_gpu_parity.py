@@ -466,6 +466,58 @@ def test_chol_single_block_bits_banded(n):
     assert np.array_equal(l.view(np.uint64), l_or.view(np.uint64))
 
 
+@pytest.mark.parametrize("n", [1, 40, 64, 150, 520])
+def test_chol_f32_vs_oracle(n):
+    """Cholesky<f32> through rlg_chol_decompose_f32, rlg_chol_unpack_f32 and
+    rlg_chol_solve_f32: bit-identical to the oracle's FP32 restatement inside
+    one 64-column block, within FP32 rounding beyond it
+    (||L - L_oracle||max / ||L_oracle||max <= 1e-5); the solve's normwise
+    backward error, in FP64 arithmetic on the FP32 data, <= 64 n u_f32."""
+    a = wl.spd(n, 700 + n).astype(np.float32)
+    w_or = orc.chol_decompose(a, dtype=np.float32)
+    l_or = orc.chol_unpack(w_or)
+    ch = rl.Cholesky.decompose(rl.Matrix(a))
+    l = ch.unpack().to_numpy()
+    assert l.dtype == np.float32
+    if n <= 64:
+        assert np.array_equal(l.view(np.uint32), l_or.view(np.uint32))
+    else:
+        assert np.abs(l - l_or).max() / np.abs(l_or).max() <= 1e-5
+    b = wl.gaussian(n, 1, 750 + n)[:, 0].astype(np.float32)
+    x = ch.solve(rl.Vector(b, dtype=np.float32)).data()
+    assert x.dtype == np.float32
+    u32 = np.finfo(np.float32).eps
+    assert eta(a.astype(np.float64), x.astype(np.float64), b.astype(np.float64)) <= 64 * n * u32
+    x_or = orc.chol_solve(w_or, b)
+    assert np.abs(x - x_or).max() / np.abs(x_or).max() <= 1e-4
+
+
+@pytest.mark.parametrize("n", [1, 37, 64, 300])
+def test_lu_unpack_entry_point(n):
+    """rlg_lu_unpack_f64 (PartialPivLu::unpack, lu.rs:135-150) called directly:
+    L with its unit diagonal, U and the stored permutation equal the oracle's
+    unpack of the oracle's factor, bit for bit inside one panel (n <= 64) and
+    within 1e-10 of the largest entry beyond it, and P A = L U."""
+    a = wl.gaussian(n, n, 5000 + n)
+    lu = rl.PartialPivLu.decompose(rl.Matrix(a))
+    l = np.empty((n, n))
+    u = np.empty((n, n))
+    perm = np.empty(n, dtype=np.uintp)
+    st = _lib.load().rlg_lu_unpack_f64(lu._h, l.ctypes.data, u.ctypes.data, perm.ctypes.data)
+    assert st == 0, _lib.last_error()
+    lu_or, perm_or, _ = orc.lu_decompose(a)
+    l_or, u_or = orc.lu_unpack(lu_or)
+    assert np.array_equal(perm, perm_or)
+    if n <= 64:
+        assert np.array_equal(l.view(np.uint64), l_or.view(np.uint64))
+        assert np.array_equal(u.view(np.uint64), u_or.view(np.uint64))
+    else:
+        assert np.abs(l - l_or).max() <= 1e-10 and np.abs(u - u_or).max() / np.abs(u_or).max() <= 1e-10
+    assert np.allclose(np.diag(l), 1.0) and np.all(np.triu(l, 1) == 0) and np.all(np.tril(u, -1) == 0)
+    rec = orc.permute_rows(orc.perm_inverse(perm), l @ u)
+    assert np.abs(rec - a).max() <= 1e-12 * np.abs(a).max() * n
+
+
 @pytest.mark.parametrize("n", [65, 128, 200, 256, 257, 520])
 def test_chol_vs_oracle(n):
     a = wl.spd(n, 600 + n)
