@@ -72,6 +72,8 @@ def parse():
                         "(GEMM 256, LU 256 + solve, LU and Cholesky fits extrapolated to 16384, "
                         "a batched subsample) and print one JSON line; no GPU is used")
     p.add_argument("--dgemm-only", action="store_true", help="only DGEMM 8192^3 (for profiling)")
+    p.add_argument("--sgemm-only", action="store_true",
+                   help="only FP32 GEMM 8192^3, FP32 and split-FP16 matrix cores (for profiling)")
     p.add_argument("--batched-only", action="store_true",
                    help="only the 20,000-system batched solve (for profiling)")
     p.add_argument("--cholesky-only", action="store_true",
@@ -120,6 +122,9 @@ def main():
         return
     if args.dgemm_only:
         print(json.dumps(dgemm_8192(lib, torch, dev, None)), flush=True)
+        return
+    if args.sgemm_only:
+        print(json.dumps(sgemm_8192(lib, torch, dev, None)), flush=True)
         return
     if args.batched_only:
         print(json.dumps(batched(lib, torch, dev)), flush=True)
@@ -626,9 +631,48 @@ def serving_trace(lib, torch, dev, rate, seconds, seed, slots=4, mixed_pivot=1):
 
 
 def extras(lib, torch, dev, sptr):
-    """DGEMM 8192^3 and Cholesky 16384 (plain FP64) timings."""
-    out = {"dgemm_8192": dgemm_8192(lib, torch, dev, sptr)}
+    """DGEMM 8192^3, FP32 GEMM 8192^3 (FP32 and split-FP16 matrix cores) and
+    Cholesky 16384 (plain FP64) timings."""
+    out = {"dgemm_8192": dgemm_8192(lib, torch, dev, sptr), "sgemm_8192": sgemm_8192(lib, torch, dev, sptr)}
     out["cholesky_16384"] = cholesky_16384(lib, torch, dev, sptr)
+    return out
+
+
+def sgemm_8192(lib, torch, dev, sptr):
+    """C = A B at 8192^3 in FP32 (U[-1, 1] operands) two ways: the FP32
+    matrix cores (rlg_gemm_f32_dev) and the split-FP16 form
+    (rlg_gemm_f32_split_dev); best of three after a warm-up, each checked by a
+    Freivalds probe against FP64."""
+    from rulinalg_amd import _lib
+    g = torch.Generator(device=dev)
+    m = 8192
+    g.manual_seed(20)
+    a = torch.rand(m, m, dtype=torch.float32, device=dev, generator=g) * 2 - 1
+    g.manual_seed(21)
+    b = torch.rand(m, m, dtype=torch.float32, device=dev, generator=g) * 2 - 1
+    c = torch.empty_like(a)
+    z = torch.randn(m, 4, dtype=torch.float64, device=dev)
+    ref = a.double() @ (b.double() @ z)
+    scale = a.double().abs().sum(1).max().item() * b.double().abs().sum(1).max().item() * z.abs().max().item()
+    out = {}
+    for key, name in (("fp32_mfma", "rlg_gemm_f32_dev"), ("split_fp16", "rlg_gemm_f32_split_dev")):
+        fn = getattr(lib, name)
+        torch.cuda.synchronize()
+        times = []
+        for it in range(4):
+            t0 = time.perf_counter()
+            assert fn(m, m, m, 1.0, a.data_ptr(), m, b.data_ptr(), m, 0.0, c.data_ptr(), m, sptr) == 0, \
+                _lib.last_error()
+            assert lib.rlg_synchronize() == 0
+            if it:
+                times.append((time.perf_counter() - t0) * 1e3)
+        err = (c.double() @ z - ref).abs().max().item() / scale
+        assert err <= 1e-6, f"SGEMM 8192 ({key}) Freivalds probe {err:.3e}"
+        tf = 2 * m ** 3 / (min(times) / 1e3) / 1e12
+        out[key] = {"ms": round(min(times), 3), "tflops": round(tf, 2), "freivalds_rel": err}
+        log(f"SGEMM 8192 {key}: {min(times):.2f} ms, {tf:.1f} TF/s")
+    out["split_over_fp32_peak"] = round(out["split_fp16"]["tflops"] / 157.3, 3)
+    del a, b, c
     return out
 
 

@@ -82,6 +82,16 @@ int rlg_gemm_f64_dev(size_t m, size_t k, size_t n, double alpha,
 int rlg_gemm_f32_dev(size_t m, size_t k, size_t n, float alpha,
                      const float *a, size_t lda, const float *b, size_t ldb,
                      float beta, float *c, size_t ldc, void *stream);
+/* The same FP32 product on the FP16 matrix cores (no counterpart in the
+ * reference; it stands behind the same `&a * &b` for f32, mat_mul.rs:57-67,
+ * where the caller accepts FP32-level rather than FP32-exact rounding):
+ * every row of A and column of B is scaled by a power of two and split as
+ * hi + lo FP16, and C accumulates hi*hi + hi*lo + lo*hi in FP32.  Normwise
+ * error of order 2^-22 ||A|| ||B|| plus FP32 accumulation.  Allocates a
+ * packing workspace of about 8 (m + n) k bytes per call. */
+int rlg_gemm_f32_split_dev(size_t m, size_t k, size_t n, float alpha,
+                           const float *a, size_t lda, const float *b, size_t ldb,
+                           float beta, float *c, size_t ldc, void *stream);
 
 /* y = A * v for an m x n A: `&Matrix * &Vector` (src/matrix/impl_ops.rs:298-315).
  * v_len != n returns RLG_E_SHAPE (the assert at impl_ops.rs:304). */

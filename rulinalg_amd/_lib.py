@@ -78,6 +78,7 @@ SIGNATURES = {
     "rlg_gemm_f32": (i32, [sz, sz, vp, sz, sz, sz, vp, sz, vp, sz]),
     "rlg_gemm_f64_dev": (i32, [sz, sz, sz, dbl, vp, sz, vp, sz, dbl, vp, sz, vp]),
     "rlg_gemm_f32_dev": (i32, [sz, sz, sz, flt, vp, sz, vp, sz, flt, vp, sz, vp]),
+    "rlg_gemm_f32_split_dev": (i32, [sz, sz, sz, flt, vp, sz, vp, sz, flt, vp, sz, vp]),
     "rlg_matvec_f64": (i32, [sz, sz, vp, sz, sz, vp, vp]),
     "rlg_solve_u_triangular_f64": (i32, [sz, sz, vp, sz, sz, sz, vp, sz, vp, sz]),
     "rlg_solve_l_triangular_f64": (i32, [sz, sz, vp, sz, sz, sz, vp, sz, vp, sz]),

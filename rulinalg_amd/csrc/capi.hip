@@ -301,6 +301,22 @@ int rlg_gemm_f32_dev(size_t m, size_t k, size_t n, float alpha, const float *a, 
     return gemm_f32(pick_stream(d, stream), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
 }
 
+int rlg_gemm_f32_split_dev(size_t m, size_t k, size_t n, float alpha, const float *a, size_t lda,
+                           const float *b, size_t ldb, float beta, float *c, size_t ldc, void *stream) {
+    Device *d;
+    RLG_TRY(current_device(&d));
+    if (lda < k || ldb < n || ldc < n) { set_error("row stride smaller than the column count"); return RLG_E_INVALID_ARG; }
+    if (m == 0 || n == 0) return RLG_OK;
+    if (!a || !b || !c) { set_error("null operand"); return RLG_E_INVALID_ARG; }
+    hipStream_t s = pick_stream(d, stream);
+    if (k == 0) return gemm_f32(s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);  // C = beta C
+    const size_t bytes = gemm_lowp_pack_bytes(m, n, k);
+    PoolBuf pack;
+    RLG_TRY(pack.alloc(bytes, s));
+    return gemm_lowp(s, RLG_PREC_F16X3, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, false, false, pack.p,
+                     bytes);
+}
+
 int rlg_matvec_f64(size_t m, size_t n, const double *a, size_t lda, size_t v_len, const double *v,
                    double *y) {
     if (v_len != n) {

@@ -93,6 +93,31 @@ def test_dgemm_8192_freivalds(lib):
     assert r / (inf_norm(a) * inf_norm(b) * z.abs().max().item()) <= 1e-13
 
 
+@pytest.mark.parametrize("m,k,n", [(300, 70, 257), (4096, 512, 4096), (5000, 700, 4200)])
+def test_gemm_f32_split(lib, m, k, n):
+    """rlg_gemm_f32_split_dev (FP32 product on the FP16 matrix cores, operands
+    split as hi + lo): C = C0 - A B, checked against the FP64 product.  The
+    normwise error ||C - C_exact||_F / (||A||_F ||B||_F + ||C0||_F) stays at
+    FP32 level (<= 1e-6; the dropped lo*lo term is below 2^-22 of the scaled
+    operands) and within 4x the FP32 matrix-core product's error plus 1e-7.
+    The last shape is ragged in m, n and k (k padded to a multiple of 64)."""
+    a = torch.rand(m, k, dtype=torch.float32, device=dev(), generator=gen(60)) * 2 - 1
+    b = torch.rand(k, n, dtype=torch.float32, device=dev(), generator=gen(61)) * 2 - 1
+    c0 = torch.rand(m, n, dtype=torch.float32, device=dev(), generator=gen(62)) * 2 - 1
+    exact = c0.double() - a.double() @ b.double()
+    scale = (a.double().norm() * b.double().norm() + c0.double().norm()).item()
+    errs = {}
+    for name in ("rlg_gemm_f32_split_dev", "rlg_gemm_f32_dev"):
+        c = c0.clone()
+        torch.cuda.synchronize()
+        st = getattr(lib, name)(m, k, n, -1.0, a.data_ptr(), k, b.data_ptr(), n, 1.0, c.data_ptr(), n, None)
+        assert st == 0, _lib.last_error()
+        assert lib.rlg_synchronize() == 0
+        errs[name] = (c.double() - exact).norm().item() / scale
+    assert errs["rlg_gemm_f32_split_dev"] <= 1e-6, errs
+    assert errs["rlg_gemm_f32_split_dev"] <= 4 * errs["rlg_gemm_f32_dev"] + 1e-7, errs
+
+
 def test_lu_16384_16rhs(lib):
     n, nrhs = 16384, 16
     a0 = torch.randn(n, n, dtype=torch.float64, device=dev(), generator=gen(12))
