@@ -17,6 +17,7 @@ rulinalg's PartialPivLu::decompose + solve, timed on a bounded sample and
 scaled by n^3).
 """
 import argparse
+import collections
 import ctypes as C
 import json
 import os
@@ -623,6 +624,19 @@ def serving_trace(lib, torch, dev, rate, seconds, seed, slots=4, mixed_pivot=1):
         del loop
         torch.cuda.synchronize()
         s = serving.summarize(reqs)
+        bad = [r for r in reqs if r.eta is not None and r.eta > 1e-12]
+        if bad:
+            # the same systems again, one at a time: an error that goes away
+            # when nothing runs beside it came from the overlap
+            s["eta_above_1e-12"] = {"count": len(bad), "paths": dict(collections.Counter(r.path for r in bad))}
+            again = [serving.Request(r.rid, r.kind, r.n, 1, r.a, r.b, torch.empty_like(r.b), 0.0)
+                     for r in sorted(bad, key=lambda r: -r.eta)[:8]]
+            solo = serving.ServingLoop(lib, torch, dev, mixed=mixed, slots=1, mixed_pivot=mixed_pivot)
+            for r in again:
+                solo.run([r])
+            solo.close()
+            torch.cuda.synchronize()
+            s["eta_above_1e-12"]["alone"] = [{"rid": r.rid, "n": r.n, "eta": r.eta} for r in again]
         out["mixed" if mixed else "plain"] = s
         log(f"serving {rate}/s {'mixed' if mixed else 'plain'}: {s}")
     del specs

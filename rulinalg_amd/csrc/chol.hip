@@ -64,8 +64,13 @@ __device__ __forceinline__ T dot8_reg(const T (&x)[CNB], const T *l) {
 // left-looking recurrence (cholesky.rs:135-163): a[k][j] -= dot(a[k][0..j],
 // a[j][0..j]) for k >= j in utils::dot order, the |d| < eps and d < 0
 // checks, then division by sqrt(d).  Every workgroup does this redundantly
-// (the same arithmetic, so the same values); workgroup 0 writes the block,
-// the divisors and the failure index.  Wave 1 solves the workgroup's 64
+// (the same arithmetic, so the same values).  The block is written back in
+// place, so it may be written only once every workgroup has read it: the
+// workgroups count themselves in `arrive` once their loads are done, and the
+// last to arrive writes the block and the divisors (a workgroup may start
+// long after the others when other streams' kernels hold the CUs, and then
+// must still read the unfactored block).  Workgroup 0 writes the failure
+// index, which no workgroup reads.  Wave 1 solves the workgroup's 64
 // rows below the block, x L11^T = b, with the same recurrence,
 // x[c] = (b[c] - dot(x[0..c], L11[c][0..c])) / divisor[c], one column
 // behind wave 0: column c needs only row c of L11 and divisor c, both final
@@ -76,11 +81,13 @@ __device__ __forceinline__ T dot8_reg(const T (&x)[CNB], const T *l) {
 template <class T>
 __global__ __launch_bounds__(128) void k_chol_block(T *__restrict__ A, long lda, int j0, int jb,
                                                     int r0, int n, T *__restrict__ divisor,
-                                                    int64_t *__restrict__ info) {
+                                                    int64_t *__restrict__ info,
+                                                    unsigned *__restrict__ arrive) {
 #pragma clang fp contract(off)
     __shared__ T a[CNB][CNB + 1];
     __shared__ T X[64][CNB + 1];
     __shared__ T sdiv[CNB];
+    __shared__ int last;  // this workgroup is the last to have read the block
     const int tid = threadIdx.x, t = tid & 63;
     const bool dw = tid < 64;  // wave 0: diagonal block; wave 1: rows below
     const int rb = r0 + (int)blockIdx.x * 64;
@@ -101,6 +108,10 @@ __global__ __launch_bounds__(128) void k_chol_block(T *__restrict__ A, long lda,
         }
     }
     __syncthreads();
+    // every load of this workgroup has returned (its values are in LDS)
+    if (tid == 0)
+        last = __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+               gridDim.x - 1;
     T x[CNB];  // wave 0: row t of the diagonal block; wave 1: row rb + t
 #pragma unroll
     for (int c = 0; c < CNB; ++c) x[c] = dw ? (c <= t ? a[t][c] : T(0)) : X[t][c];
@@ -144,7 +155,7 @@ __global__ __launch_bounds__(128) void k_chol_block(T *__restrict__ A, long lda,
 #pragma unroll
         for (int c = 0; c < CNB; ++c)
             if (c == t && c < jend) a[t][t] = x[c];
-        if (blockIdx.x == 0) {
+        if (last) {  // set before the loop's first barrier (jb >= 1)
             if (t < jend) divisor[j0 + t] = sdiv[t];
 #pragma unroll 8
             for (int r = 0; r < jb; ++r)
@@ -161,13 +172,19 @@ __global__ __launch_bounds__(128) void k_chol_block(T *__restrict__ A, long lda,
     }
 }
 
-size_t chol_work_bytes(size_t n, size_t elem) { return ((n + 1) * elem + 255) & ~(size_t)255; }
+// Workspace: the divisors, then one arrival counter per 64-column block.
+static size_t chol_div_bytes(size_t n, size_t elem) { return ((n + 1) * elem + 255) & ~(size_t)255; }
+size_t chol_work_bytes(size_t n, size_t elem) {
+    return chol_div_bytes(n, elem) + ((ceil_div(n, CNB) * sizeof(unsigned) + 255) & ~(size_t)255);
+}
 
 template <class T>
 static int chol_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info, void *work,
                      int lowp_prec) {
     T *divisor = static_cast<T *>(work);
+    unsigned *arrive = reinterpret_cast<unsigned *>(static_cast<char *>(work) + chol_div_bytes(n, sizeof(T)));
     RLG_TRY(fill_i64(info, 1, (int64_t)n, s));
+    RLG_TRY(fill_u32(arrive, ceil_div(n, CNB), 0u, s));
     for (size_t k0 = 0; k0 < n; k0 += CONB) {
         const size_t kb = n - k0 < (size_t)CONB ? n - k0 : CONB;
         for (size_t j0 = k0; j0 < k0 + kb; j0 += CNB) {
@@ -177,7 +194,8 @@ static int chol_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info, v
                 RLG_PROF("chol_block", s);
                 const size_t below = n - r0;  // rows under the diagonal block
                 hipLaunchKernelGGL(k_chol_block<T>, dim3(below ? ceil_div(below, 64) : 1), dim3(128), 0,
-                                   s, a, (long)lda, (int)j0, (int)jb, (int)r0, (int)n, divisor, info);
+                                   s, a, (long)lda, (int)j0, (int)jb, (int)r0, (int)n, divisor, info,
+                                   arrive + j0 / CNB);
                 RLG_LAUNCHED();
             }
             if (r0 < k0 + kb) {

@@ -389,10 +389,13 @@ def summarize(reqs):
     lat = np.array([r.latency_ms for r in ok]) if ok else np.array([0.0])
     first = min(r.t_arr for r in reqs)
     last = max(r.done for r in reqs if r.done is not None)
-    etas = [r.eta for r in ok if r.eta is not None]
+    checked = [r for r in ok if r.eta is not None]
+    worst = max(checked, key=lambda r: r.eta) if checked else None
     paths = collections.Counter(r.path for r in reqs)
     return {"requests": len(reqs), "completed": len(ok), "failed": len(reqs) - len(ok),
             "solves_per_s": round(len(ok) / (last - first), 1),
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "backward_error_max": max(etas) if etas else None, "eta_checked": len(etas),
+            "backward_error_max": worst.eta if worst else None, "eta_checked": len(checked),
+            "worst": ({"rid": worst.rid, "kind": "lu" if worst.kind == KIND_LU else "chol", "n": worst.n,
+                       "path": worst.path} if worst else None),
             "fallbacks": sum(r.fell_back for r in ok), "paths": dict(paths)}
