@@ -145,10 +145,13 @@ class ServingLoop:
             self._drop(victim)
         return True
 
-    def _drop(self, fid):
-        h, kind, _n, _w, _a, nbytes = self.resident.pop(fid)
+    def _free(self, entry):
+        h, kind, _n, _w, _a, nbytes = entry
         (self.lib.rlg_lu_free if kind == KIND_LU else self.lib.rlg_chol_free)(h)
         self.resident_bytes -= nbytes
+
+    def _drop(self, fid):
+        self._free(self.resident.pop(fid))
 
     def evict(self, fid):
         if fid in self.resident:
@@ -272,6 +275,7 @@ class ServingLoop:
         keeping = {}          # keep_id -> in-flight request
         in_flight_bytes = 0
         in_use = collections.Counter()  # resident id -> requests using it
+        retired = []          # (id, entry): factors replaced under their id while still in use
         outstanding = len(requests)
         failure = None
 
@@ -301,10 +305,21 @@ class ServingLoop:
                         kept, r._kept = r._kept, None
                         if kept is not None:
                             if r.keep_id in self.resident:
-                                self._drop(r.keep_id)
+                                if in_use[r.keep_id] > 0:  # freed once its solves are done
+                                    retired.append((r.keep_id, self.resident.pop(r.keep_id)))
+                                else:
+                                    self._drop(r.keep_id)
                             nbytes = 16 * r.n * r.n  # the factor and the original matrix
                             self.resident[r.keep_id] = (kept[0], r.kind, r.n, kept[1], r.a, nbytes)
                             self.resident_bytes += nbytes
+            if retired:
+                keep = []
+                for fid, entry in retired:
+                    if in_use[fid] > 0:
+                        keep.append((fid, entry))
+                    else:
+                        self._free(entry)
+                retired = keep
             if failure is not None:
                 raise failure
             # ---- reuse requests whose factor has since been computed
@@ -379,6 +394,8 @@ class ServingLoop:
                     item = None
                 if item is not None:
                     self.completions.put(item)  # handled at the top of the loop
+        for _fid, entry in retired:  # every request is done: nothing uses them
+            self._free(entry)
         return requests
 
 

@@ -171,3 +171,41 @@ def test_mixed_mode_routes_large_lu_to_the_mixed_solve():
     loop.run(reqs)
     assert [r.path for r in reqs] == ["single_mixed", "single", "batched_mixed"]
     assert ("mixed", 600) in lib.calls
+
+
+def test_replaced_factor_outlives_its_solves():
+    """A request that keeps its factor under an id that is already resident
+    replaces it; a reuse solve still running on the old factor keeps it: the
+    old handle is freed only after that solve has finished."""
+    import threading as th
+
+    gate = th.Event()
+
+    class GatedLib(StubLib):
+        def rlg_lu_solve_f64_dev(self, h, nrhs, b, ldb, x, ldx, stream):
+            hv = h.value if hasattr(h, "value") else h
+            # the reuse request's solve: the second one on the first factor
+            if hv == 1 and ("solve_1",) in self.calls and not gate.is_set():
+                self.calls.append(("solve_start", hv))
+                gate.wait(5)
+                self.calls.append(("solve_done", hv))
+                return 0
+            if hv == 1:
+                self.calls.append(("solve_1",))
+            return super().rlg_lu_solve_f64_dev(h, nrhs, b, ldb, x, ldx, stream)
+
+    lib = GatedLib()
+    loop = serving.ServingLoop(lib, torch_cpu(), "cpu")
+    first = req(0, serving.KIND_LU, 600, keep_id=7)
+    reuse = req(1, serving.KIND_LU, 600, arrival=0.05, reuse_id=7)
+    second = req(2, serving.KIND_LU, 600, arrival=0.10, keep_id=7)
+    timer = th.Timer(0.6, gate.set)
+    timer.start()
+    loop.run([first, reuse, second])
+    timer.join()
+    assert all(r.status == 0 for r in (first, reuse, second))
+    assert ("solve_done", 1) in lib.calls
+    # the first factor (handle 1) is freed, and only after the solve on it ended
+    assert lib.calls.index(("free", 1)) > lib.calls.index(("solve_done", 1))
+    assert 7 in loop.resident and loop.resident[7][0].value == 2
+    loop.close()
