@@ -35,7 +35,9 @@ resident under that id, with its original matrix for the backward error; a
 later request with ``reuse_id`` solves new right-hand sides against it
 without refactoring (several such solves may run at once: the handle is
 immutable).  A reuse request that arrives while the factor it names is
-still being computed waits for it.
+still being computed waits for it.  Reuse solves take a free slot worker
+before any queued factorization: each is two triangular solves, far shorter
+than the factorizations ahead of it.
 
 The reference has no serving layer (rulinalg is a library of synchronous
 calls); the loop serves its calls: PartialPivLu::decompose + solve
@@ -64,7 +66,7 @@ class Request:
     request); ``x`` (nrhs*n) receives the solution."""
 
     __slots__ = ("rid", "kind", "n", "nrhs", "a", "b", "x", "arrival", "keep_id", "reuse_id",
-                 "check", "status", "done", "eta", "path", "fell_back", "t_arr", "need", "_kept")
+                 "check", "status", "done", "eta", "path", "fell_back", "t_arr", "t_start", "need", "_kept")
 
     def __init__(self, rid, kind, n, nrhs, a, b, x, arrival, keep_id=None, reuse_id=None, check=True):
         self.rid, self.kind, self.n, self.nrhs = rid, kind, n, nrhs
@@ -72,6 +74,7 @@ class Request:
         self.keep_id, self.reuse_id, self.check = keep_id, reuse_id, check
         self.status, self.done, self.eta, self.path, self.fell_back = None, None, None, None, 0
         self.t_arr = None  # arrival on the host clock, set by ServingLoop.run
+        self.t_start = None  # when a worker started on it
         self.need = 0      # device bytes held while in flight
         self._kept = None  # (handle, factor tensor) of a kept factor, for the dispatcher
 
@@ -97,6 +100,9 @@ class _Worker(threading.Thread):
             if job is None:
                 return
             fn, arg = job
+            t = time.perf_counter()
+            for r in (arg if isinstance(arg, list) else [arg[0]]):
+                r.t_start = t
             try:
                 fn(self, arg)
                 err = None
@@ -137,9 +143,11 @@ class ServingLoop:
             return 0
         return 8 * r.n * r.n + 8 * r.nrhs * r.n + (16 * r.n * r.n if r.keep_id is not None else 0)
 
-    def _evict_for(self, need, protect):
+    def _evict_for(self, need, protected):
+        """Evicts least-recently-used resident factors for which
+        `protected(id)` is false until `need` more bytes fit."""
         while self.resident_bytes + need > self.budget:
-            victim = next((k for k in self.resident if k not in protect), None)
+            victim = next((k for k in self.resident if not protected(k)), None)
             if victim is None:
                 return False
             self._drop(victim)
@@ -270,7 +278,9 @@ class ServingLoop:
             r.t_arr = t0 + r.arrival
         pending = collections.deque(sorted(requests, key=lambda r: r.t_arr))
         small_q = []          # admitted one-shot systems of order <= 512
-        single_q = collections.deque()  # admitted per-system requests
+        single_q = collections.deque()  # admitted per-system factorizations
+        reuse_q = collections.deque()   # admitted solves on a resident factor: served first
+        queued_reuse = collections.Counter()  # resident id -> queued requests that will use it
         waiting_reuse = []    # reuse requests whose factor is still being computed
         keeping = {}          # keep_id -> in-flight request
         in_flight_bytes = 0
@@ -328,8 +338,10 @@ class ServingLoop:
                 if r.reuse_id in keeping:
                     still.append(r)
                 elif r.reuse_id in self.resident:
-                    single_q.append(r)
+                    reuse_q.append(r)
                 else:
+                    queued_reuse[r.reuse_id] -= 1
+                    in_flight_bytes -= r.need
                     finish_now(r, 1, "reuse")  # InvalidArg: the factor is not resident
             waiting_reuse = still
             # ---- admission in arrival order
@@ -341,11 +353,10 @@ class ServingLoop:
                     pending.popleft()
                     finish_now(r, RLG_E_OOM, "rejected")
                     continue
-                protect = set(k for k, c in in_use.items() if c > 0)
-                protect |= {q.reuse_id for q in list(single_q) + waiting_reuse if q.reuse_id is not None}
-                if r.reuse_id is not None:
-                    protect.add(r.reuse_id)
-                if in_flight_bytes + need > self.budget or not self._evict_for(in_flight_bytes + need, protect):
+                def protected(k, rid=r.reuse_id):
+                    return in_use[k] > 0 or queued_reuse[k] > 0 or k == rid
+                if in_flight_bytes + need > self.budget or not self._evict_for(in_flight_bytes + need,
+                                                                                 protected):
                     break  # wait for work in flight to finish
                 pending.popleft()
                 r.need = need
@@ -353,8 +364,10 @@ class ServingLoop:
                 if r.reuse_id is not None:
                     if r.reuse_id in keeping:
                         waiting_reuse.append(r)
+                        queued_reuse[r.reuse_id] += 1
                     elif r.reuse_id in self.resident:
-                        single_q.append(r)
+                        reuse_q.append(r)
+                        queued_reuse[r.reuse_id] += 1
                     else:
                         in_flight_bytes -= need
                         finish_now(r, 1, "reuse")
@@ -370,17 +383,18 @@ class ServingLoop:
                 self.batch_worker.jobs.put((ServingLoop._job_batch.__get__(self), small_q))
                 small_q = []
             for w in self.slot_workers:
-                if not single_q:
+                if not reuse_q and not single_q:
                     break
                 if w.busy:
                     continue
-                r = single_q.popleft()
+                r = reuse_q.popleft() if reuse_q else single_q.popleft()
                 resident = None
                 if r.reuse_id is not None:
                     self.resident.move_to_end(r.reuse_id)
                     h, kind, _rn, _w, a, _nb = self.resident[r.reuse_id]
                     resident = (h, kind, a)
                     in_use[r.reuse_id] += 1
+                    queued_reuse[r.reuse_id] -= 1
                 w.busy = True
                 w.jobs.put((ServingLoop._job_single.__get__(self), (r, resident)))
             # ---- wait for the next event: a completion or the next arrival
@@ -415,4 +429,23 @@ def summarize(reqs):
             "backward_error_max": worst.eta if worst else None, "eta_checked": len(checked),
             "worst": ({"rid": worst.rid, "kind": "lu" if worst.kind == KIND_LU else "chol", "n": worst.n,
                        "path": worst.path} if worst else None),
-            "fallbacks": sum(r.fell_back for r in ok), "paths": dict(paths)}
+            "fallbacks": sum(r.fell_back for r in ok), "paths": dict(paths),
+            "by_path": _by_path(ok)}
+
+
+def _by_path(ok):
+    """Per path: requests, and p50 / p99 of the time queued before a worker
+    took the request and of the time from then to its result (ms)."""
+    import numpy as np
+    out = {}
+    for path in sorted({r.path for r in ok}):
+        rs = [r for r in ok if r.path == path and r.t_start is not None]
+        if not rs:
+            continue
+        q = np.array([(r.t_start - r.t_arr) * 1e3 for r in rs])
+        sv = np.array([(r.done - r.t_start) * 1e3 for r in rs])
+        out[path] = {"requests": len(rs), "queue_p50_ms": round(float(np.percentile(q, 50)), 2),
+                     "queue_p99_ms": round(float(np.percentile(q, 99)), 2),
+                     "service_p50_ms": round(float(np.percentile(sv, 50)), 2),
+                     "service_p99_ms": round(float(np.percentile(sv, 99)), 2)}
+    return out

@@ -209,3 +209,28 @@ def test_replaced_factor_outlives_its_solves():
     assert lib.calls.index(("free", 1)) > lib.calls.index(("solve_done", 1))
     assert 7 in loop.resident and loop.resident[7][0].value == 2
     loop.close()
+
+
+def test_reuse_solves_go_before_queued_factorizations():
+    """With one slot worker busy, a reuse solve admitted after a large
+    factorization is still dispatched before it."""
+    import time as _t
+
+    class SlowLib(StubLib):
+        def rlg_lu_decompose_f64_dev(self, n, a, lda, stream, out):
+            if n == 600:
+                _t.sleep(0.3)  # the slot stays busy while the next two arrive
+            return super().rlg_lu_decompose_f64_dev(n, a, lda, stream, out)
+
+    lib = SlowLib()
+    loop = serving.ServingLoop(lib, torch_cpu(), "cpu", slots=1)
+    keep = req(0, serving.KIND_LU, 600, keep_id=5)
+    big = req(1, serving.KIND_LU, 700, arrival=0.05)
+    again = req(2, serving.KIND_LU, 600, arrival=0.1, reuse_id=5)
+    loop.run([keep, big, again])
+    assert all(r.status == 0 for r in (keep, big, again))
+    assert again.path == "reuse"
+    solves = [i for i, c in enumerate(lib.calls) if c == ("lu_solve", 600)]
+    assert len(solves) == 2  # the keep request's own solve, then the reuse solve
+    assert solves[1] < lib.calls.index(("lu", 700))
+    loop.close()
