@@ -258,6 +258,112 @@ __global__ __launch_bounds__(GTHREADS, 2) void k_gemm(
     }
 }
 
+// Depth K <= 64 (the LU and Cholesky updates inside an outer block, which
+// are many small launches): the whole of op(B) and A fits one LDS stage, so
+// every load of A, B and C is issued before the first wait and a workgroup
+// pays one memory latency, where k_gemm pays one per stage plus the
+// epilogue's.  Same fragments, k order and epilogue as k_gemm, so the
+// results are identical.  Needs 16-byte vectors (the VEC conditions).
+constexpr int SHK = 64;
+template <class T, bool TRANSB>
+__global__ __launch_bounds__(GTHREADS) void k_gemm_shallow(int M, int N, int K, T alpha,
+                                                           const T *__restrict__ A, long lda,
+                                                           const T *__restrict__ B, long ldb, T beta,
+                                                           T *__restrict__ C, long ldc, int tiles_m,
+                                                           int tiles_n) {
+    using Acc = typename Mfma<T>::acc;
+    constexpr int VL = 16 / sizeof(T);
+    constexpr int VPER = SHK * GBM / GTHREADS / VL;  // 16-byte loads per thread per operand
+    typedef T vT __attribute__((ext_vector_type(VL)));
+    __shared__ T As[SHK][GBM + 1];
+    __shared__ __attribute__((aligned(16))) T Bs[SHK][GBN + (TRANSB ? 1 : VL)];
+
+    int tm, tn;
+    tile_coords<false>(tiles_m, tiles_n, tm, tn);
+    const int m0 = tm * GBM, n0 = tn * GBN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    vT rav[VPER], rbv[VPER];
+#pragma unroll
+    for (int q = 0; q < VPER; ++q) {
+        const int e = q * GTHREADS + tid;
+        {   // A tile, 128 x SHK in SHK / VL vectors per row
+            const int i = e / (SHK / VL), gk = (e % (SHK / VL)) * VL, gi = m0 + i;
+            rav[q] = (gi < M && gk < K) ? *reinterpret_cast<const vT *>(A + (long)gi * lda + gk) : vT{};
+        }
+        if (!TRANSB) {  // B tile, SHK x 128 in 128 / VL vectors per row
+            const int gk = e / (GBN / VL), gj = n0 + (e % (GBN / VL)) * VL;
+            rbv[q] = (gk < K && gj < N) ? *reinterpret_cast<const vT *>(B + (long)gk * ldb + gj) : vT{};
+        } else {        // B^T given as N x K, vectors along k
+            const int gj = n0 + e / (SHK / VL), gk = (e % (SHK / VL)) * VL;
+            rbv[q] = (gj < N && gk < K) ? *reinterpret_cast<const vT *>(B + (long)gj * ldb + gk) : vT{};
+        }
+    }
+    T cin[4][4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wm + i * 16 + Mfma<T>::row(lk, r);
+                const int gj = n0 + wn + j * 16 + lr;
+                cin[i][j][r] = (beta != T(0) && gi < M && gj < N) ? C[(long)gi * ldc + gj] : T(0);
+            }
+#pragma unroll
+    for (int q = 0; q < VPER; ++q) {
+        const int e = q * GTHREADS + tid;
+        const int i = e / (SHK / VL), kb = (e % (SHK / VL)) * VL;
+#pragma unroll
+        for (int v = 0; v < VL; ++v) As[kb + v][i] = rav[q][v];
+        if (!TRANSB) {
+            *reinterpret_cast<vT *>(&Bs[e / (GBN / VL)][(e % (GBN / VL)) * VL]) = rbv[q];
+        } else {
+#pragma unroll
+            for (int v = 0; v < VL; ++v) Bs[kb + v][i] = rbv[q][v];
+        }
+    }
+    __syncthreads();
+
+    Acc acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Acc{0, 0, 0, 0};
+    // as many k-steps as k_gemm's whole stages take (rows from K on are
+    // zero), so that even the signs of zero results agree with it
+    constexpr int BK = StageK<T>::value;
+    const int ksteps = (K + BK - 1) / BK * (BK / 4);
+    for (int ks = 0; ks < ksteps; ++ks) {
+        T af[4], bf[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) af[i] = As[ks * 4 + lk][wm + i * 16 + lr];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bf[j] = Bs[ks * 4 + lk][wn + j * 16 + lr];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::op(af[i], bf[j], acc[i][j]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wm + i * 16 + Mfma<T>::row(lk, r);
+                const int gj = n0 + wn + j * 16 + lr;
+                if (gi < M && gj < N) {
+                    T v = alpha * acc[i][j][r];
+                    if (beta != T(0)) v += beta * cin[i][j][r];
+                    C[(long)gi * ldc + gj] = v;
+                }
+            }
+}
+
 // ---------------------------------------------------------------- low precision
 
 template <class H> struct LowpMfma;
@@ -852,6 +958,16 @@ static int gemm_impl(hipStream_t s, const char *cls, size_t m, size_t n, size_t 
     const bool vec = k % VL == 0 && lda % VL == 0 && ldb % VL == 0 && al16(a) && al16(b) &&
                      (trans_b || n % VL == 0);
     const bool cp = vec && sizeof(T) == 4 && k <= 1024;
+    if (vec && !lower_only && k > 0 && k <= (size_t)SHK) {
+        if (trans_b)
+            hipLaunchKernelGGL((k_gemm_shallow<T, true>), grid, block, 0, s, (int)m, (int)n, (int)k, alpha,
+                               a, (long)lda, b, (long)ldb, beta, c, (long)ldc, tiles_m, tiles_n);
+        else
+            hipLaunchKernelGGL((k_gemm_shallow<T, false>), grid, block, 0, s, (int)m, (int)n, (int)k, alpha,
+                               a, (long)lda, b, (long)ldb, beta, c, (long)ldc, tiles_m, tiles_n);
+        RLG_LAUNCHED();
+        return RLG_OK;
+    }
 #define RLG_GEMM_LAUNCH(TB, LO, V, CPF)                                                               \
     hipLaunchKernelGGL((k_gemm<T, TB, LO, V, CPF>), grid, block, 0, s, (int)m, (int)n, (int)k, alpha, \
                        a, (long)lda, b, (long)ldb, beta, c, (long)ldc, tiles_m, tiles_n)
