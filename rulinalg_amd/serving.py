@@ -10,12 +10,9 @@ Admitted requests go to workers, each a host thread with its own HIP stream:
 
 * one-shot systems of order <= 512 collect in a queue that the batch worker
   empties whenever it is idle: all of them run in ONE persistent batched
-  launch (``rlg_batch_solve_f64_dev``), so their work overlaps on the
-  device, and systems arriving meanwhile form the next launch.  Mixed mode
-  uses the same FP64 launch: at these orders the factorization is bound by
-  latency, not arithmetic, so the low-precision batched form
-  (``rlg_batch_solve_mixed_dev``) only adds its refinement (it measured about
-  half the FP64 form's rate on the 20,000-system benchmark batch);
+  launch (``rlg_batch_solve_f64_dev``, or ``rlg_batch_solve_mixed_dev`` in
+  mixed mode), so their work overlaps on the device, and systems arriving
+  meanwhile form the next launch;
 * larger systems, and requests that keep or reuse a resident factor, go to
   the first free slot worker, which runs the per-system path
   (``rlg_{lu,chol}_decompose_f64_dev`` + ``_solve_f64_dev``, or
@@ -190,14 +187,14 @@ class ServingLoop:
         items = (_lib.BatchItem * count)()
         sp = self._sptr(worker)
         with torch.cuda.stream(worker.stream):
-            # the kernel factors in place, so it works on a copy of A; the
-            # solution overwrites the copy of B
+            # plain: the kernel factors in place, so it works on a copy of A;
+            # the solution overwrites the copy of B in both forms
             bcat = torch.cat([r.b for r in reqs])
-            acat = torch.cat([r.a for r in reqs])
+            acat = torch.cat([r.a for r in reqs]) if not self.mixed else None
             piv = torch.empty(sum(r.n for r in reqs), dtype=torch.int64, device=self.dev)
             boff = aoff = poff = 0
             for i, r in enumerate(reqs):
-                pa = acat.data_ptr() + 8 * aoff
+                pa = r.a.data_ptr() if self.mixed else acat.data_ptr() + 8 * aoff
                 items[i] = _lib.BatchItem(r.kind, r.n, r.nrhs, -1, pa, bcat.data_ptr() + 8 * boff,
                                           piv.data_ptr() + 8 * poff)
                 aoff += r.n * r.n
@@ -206,7 +203,11 @@ class ServingLoop:
             host = torch.frombuffer(bytearray(C.string_at(C.addressof(items), C.sizeof(items))),
                                     dtype=torch.uint8)
             dev_items = host.to(self.dev)
-            st = lib.rlg_batch_solve_f64_dev(count, dev_items.data_ptr(), sp)
+            fb = C.c_size_t(0)
+            if self.mixed:
+                st = lib.rlg_batch_solve_mixed_dev(count, dev_items.data_ptr(), 30, self.tol, C.byref(fb), sp)
+            else:
+                st = lib.rlg_batch_solve_f64_dev(count, dev_items.data_ptr(), sp)
             if st != 0:
                 raise RuntimeError(_lib.last_error())
             boff = 0
@@ -219,7 +220,7 @@ class ServingLoop:
         for i, r in enumerate(reqs):
             r.status = items[i].status
             r.done = t
-            r.path = "batched"
+            r.path = "batched_mixed" if self.mixed else "batched"
         for r in reqs:
             if r.check and r.status == 0:
                 r.eta = self._eta(worker, r.n, r.a.data_ptr(), r.nrhs, r.b.data_ptr(), r.x.data_ptr())

@@ -169,7 +169,7 @@ def test_mixed_mode_routes_large_lu_to_the_mixed_solve():
     loop = serving.ServingLoop(lib, torch_cpu(), "cpu", mixed=True)
     reqs = [req(0, serving.KIND_LU, 600), req(1, serving.KIND_CHOL, 600), req(2, serving.KIND_LU, 64)]
     loop.run(reqs)
-    assert [r.path for r in reqs] == ["single_mixed", "single", "batched"]  # small: the FP64 batched form
+    assert [r.path for r in reqs] == ["single_mixed", "single", "batched_mixed"]
     assert ("mixed", 600) in lib.calls
 
 
