@@ -178,6 +178,37 @@ size_t chol_work_bytes(size_t n, size_t elem) {
     return chol_div_bytes(n, elem) + ((ceil_div(n, CNB) * sizeof(unsigned) + 255) & ~(size_t)255);
 }
 
+// A second stream per host thread and device (default priority, created on
+// first use and kept): the look-ahead's trailing updates run there, so that
+// factorizations issued from different threads do not queue behind each
+// other's updates.
+static int chol_aux_stream(hipStream_t *out) {
+    static thread_local hipStream_t aux[64] = {};
+    int id = 0;
+    RLG_HIP(hipGetDevice(&id));
+    if (id < 0 || id >= 64) { set_error("device index %d out of range", id); return RLG_E_DEVICE; }
+    if (!aux[id]) RLG_HIP(hipStreamCreateWithFlags(&aux[id], hipStreamNonBlocking));
+    *out = aux[id];
+    return RLG_OK;
+}
+
+struct CholEvents {
+    hipEvent_t ready = nullptr, done = nullptr;
+    ~CholEvents() {
+        if (ready) (void)hipEventDestroy(ready);
+        if (done) (void)hipEventDestroy(done);
+    }
+};
+
+// Look-ahead: after outer block k, the trailing update is split.  The next
+// outer block's columns (all rows below it) are updated first, on the
+// caller's stream, which then factors that block; the rest of the update
+// runs meanwhile on the auxiliary stream.  The two touch disjoint columns,
+// and the caller's stream waits for the rest before its next update of the
+// same region, so every entry receives the same updates in the same order as
+// without look-ahead; the split follows the 128-wide tile grid of the
+// update (outer blocks are four tiles wide), so each entry is computed with
+// the same arithmetic as well.
 template <class T>
 static int chol_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info, void *work,
                      int lowp_prec) {
@@ -185,6 +216,15 @@ static int chol_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info, v
     unsigned *arrive = reinterpret_cast<unsigned *>(static_cast<char *>(work) + chol_div_bytes(n, sizeof(T)));
     RLG_TRY(fill_i64(info, 1, (int64_t)n, s));
     RLG_TRY(fill_u32(arrive, ceil_div(n, CNB), 0u, s));
+    const bool native = lowp_prec != RLG_PREC_F16 && lowp_prec != RLG_PREC_BF16;
+    hipStream_t s2 = nullptr;
+    CholEvents ev;
+    bool pending = false;  // an update of columns beyond the next block is running on s2
+    if (native && n > 2 * (size_t)CONB) {
+        RLG_TRY(chol_aux_stream(&s2));
+        RLG_HIP(hipEventCreateWithFlags(&ev.ready, hipEventDisableTiming));
+        RLG_HIP(hipEventCreateWithFlags(&ev.done, hipEventDisableTiming));
+    }
     for (size_t k0 = 0; k0 < n; k0 += CONB) {
         const size_t kb = n - k0 < (size_t)CONB ? n - k0 : CONB;
         for (size_t j0 = k0; j0 < k0 + kb; j0 += CNB) {
@@ -205,18 +245,37 @@ static int chol_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info, v
             }
         }
         const size_t c0 = k0 + kb;
-        if (c0 < n) {
-            if (lowp_prec == RLG_PREC_F16 || lowp_prec == RLG_PREC_BF16) {
-                RLG_TRY(gemm_lowp(s, lowp_prec, n - c0, n - c0, kb, -1.f,
-                                  reinterpret_cast<const float *>(a + c0 * lda + k0), lda,
-                                  reinterpret_cast<const float *>(a + c0 * lda + k0), lda, 1.f,
-                                  reinterpret_cast<float *>(a + c0 * lda + c0), lda, true, true));
-            } else {
-                RLG_TRY(gemm_t<T>(s, n - c0, n - c0, kb, T(-1), a + c0 * lda + k0, lda,
-                                  a + c0 * lda + k0, lda, T(1), a + c0 * lda + c0, lda, true, true));
-            }
+        if (c0 >= n) break;
+        if (pending) {  // the previous step's update of the columns beyond this block
+            RLG_HIP(hipStreamWaitEvent(s, ev.done, 0));
+            pending = false;
         }
+        const T *lk = a + c0 * lda + k0;  // this block's columns of L, rows from c0
+        if (!native) {
+            RLG_TRY(gemm_lowp(s, lowp_prec, n - c0, n - c0, kb, -1.f, reinterpret_cast<const float *>(lk),
+                              lda, reinterpret_cast<const float *>(lk), lda, 1.f,
+                              reinterpret_cast<float *>(a + c0 * lda + c0), lda, true, true));
+            continue;
+        }
+        const size_t kb2 = n - c0 < (size_t)CONB ? n - c0 : CONB;
+        if (!s2 || c0 + kb2 >= n) {
+            RLG_TRY(gemm_t<T>(s, n - c0, n - c0, kb, T(-1), lk, lda, lk, lda, T(1), a + c0 * lda + c0, lda,
+                              true, true));
+            continue;
+        }
+        // the next block's columns first (rows c0 .. n-1, lower part of its diagonal block)
+        RLG_TRY(gemm_t<T>(s, n - c0, kb2, kb, T(-1), lk, lda, lk, lda, T(1), a + c0 * lda + c0, lda, true,
+                          true));
+        // the rest on the auxiliary stream, once this block's columns are final
+        const size_t c1 = c0 + kb2;
+        RLG_HIP(hipEventRecord(ev.ready, s));
+        RLG_HIP(hipStreamWaitEvent(s2, ev.ready, 0));
+        RLG_TRY(gemm_t<T>(s2, n - c1, n - c1, kb, T(-1), a + c1 * lda + k0, lda, a + c1 * lda + k0, lda,
+                          T(1), a + c1 * lda + c1, lda, true, true));
+        RLG_HIP(hipEventRecord(ev.done, s2));
+        pending = true;
     }
+    if (pending) RLG_HIP(hipStreamWaitEvent(s, ev.done, 0));
     return RLG_OK;
 }
 
