@@ -192,6 +192,12 @@ static int chol_aux_stream(hipStream_t *out) {
     return RLG_OK;
 }
 
+// LDS the look-ahead's update workgroups reserve beyond their own 66 KB
+// (FP64): two of them no longer fit on a CU's 160 KB, one of them and a
+// block-column workgroup (67 KB) do, so the block columns find a place on
+// every CU while the update runs.
+constexpr size_t CHOL_UPDATE_PAD = 24 << 10;
+
 struct CholEvents {
     hipEvent_t ready = nullptr, done = nullptr;
     ~CholEvents() {
@@ -270,8 +276,9 @@ static int chol_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info, v
         const size_t c1 = c0 + kb2;
         RLG_HIP(hipEventRecord(ev.ready, s));
         RLG_HIP(hipStreamWaitEvent(s2, ev.ready, 0));
-        RLG_TRY(gemm_t<T>(s2, n - c1, n - c1, kb, T(-1), a + c1 * lda + k0, lda, a + c1 * lda + k0, lda,
-                          T(1), a + c1 * lda + c1, lda, true, true));
+        RLG_TRY(gemm_t_padded<T>(s2, n - c1, n - c1, kb, T(-1), a + c1 * lda + k0, lda,
+                                 a + c1 * lda + k0, lda, T(1), a + c1 * lda + c1, lda, true, true,
+                                 CHOL_UPDATE_PAD));
         RLG_HIP(hipEventRecord(ev.done, s2));
         pending = true;
     }

@@ -944,7 +944,7 @@ static int check_dims(size_t m, size_t n, size_t k) {
 template <class T>
 static int gemm_impl(hipStream_t s, const char *cls, size_t m, size_t n, size_t k, T alpha,
                      const T *a, size_t lda, const T *b, size_t ldb, T beta, T *c,
-                     size_t ldc, bool trans_b, bool lower_only) {
+                     size_t ldc, bool trans_b, bool lower_only, size_t lds_pad = 0) {
     if (m == 0 || n == 0) return RLG_OK;
     RLG_TRY(check_dims(m, n, k));
     const int tiles_m = (int)ceil_div(m, GBM), tiles_n = (int)ceil_div(n, GBN);
@@ -958,7 +958,7 @@ static int gemm_impl(hipStream_t s, const char *cls, size_t m, size_t n, size_t 
     const bool vec = k % VL == 0 && lda % VL == 0 && ldb % VL == 0 && al16(a) && al16(b) &&
                      (trans_b || n % VL == 0);
     const bool cp = vec && sizeof(T) == 4 && k <= 1024;
-    if (vec && !lower_only && k > 0 && k <= (size_t)SHK) {
+    if (vec && !lower_only && k > 0 && k <= (size_t)SHK && lds_pad == 0) {
         if (trans_b)
             hipLaunchKernelGGL((k_gemm_shallow<T, true>), grid, block, 0, s, (int)m, (int)n, (int)k, alpha,
                                a, (long)lda, b, (long)ldb, beta, c, (long)ldc, tiles_m, tiles_n);
@@ -968,9 +968,9 @@ static int gemm_impl(hipStream_t s, const char *cls, size_t m, size_t n, size_t 
         RLG_LAUNCHED();
         return RLG_OK;
     }
-#define RLG_GEMM_LAUNCH(TB, LO, V, CPF)                                                               \
-    hipLaunchKernelGGL((k_gemm<T, TB, LO, V, CPF>), grid, block, 0, s, (int)m, (int)n, (int)k, alpha, \
-                       a, (long)lda, b, (long)ldb, beta, c, (long)ldc, tiles_m, tiles_n)
+#define RLG_GEMM_LAUNCH(TB, LO, V, CPF)                                                                \
+    hipLaunchKernelGGL((k_gemm<T, TB, LO, V, CPF>), grid, block, (unsigned)lds_pad, s, (int)m, (int)n, \
+                       (int)k, alpha, a, (long)lda, b, (long)ldb, beta, c, (long)ldc, tiles_m, tiles_n)
 #define RLG_GEMM_LAUNCH_V(TB, LO)                                                   \
     do {                                                                            \
         if (cp) RLG_GEMM_LAUNCH(TB, LO, true, true);                                \
@@ -1000,6 +1000,20 @@ int gemm_f32(hipStream_t s, size_t m, size_t n, size_t k, float alpha, const flo
              bool trans_b, bool lower_only) {
     return gemm_impl<float>(s, "gemm_f32", m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
                             trans_b, lower_only);
+}
+
+int gemm_f64_padded(hipStream_t s, size_t m, size_t n, size_t k, double alpha, const double *a,
+                    size_t lda, const double *b, size_t ldb, double beta, double *c, size_t ldc,
+                    bool trans_b, bool lower_only, size_t lds_pad) {
+    return gemm_impl<double>(s, "gemm_f64", m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, trans_b,
+                             lower_only, lds_pad);
+}
+
+int gemm_f32_padded(hipStream_t s, size_t m, size_t n, size_t k, float alpha, const float *a,
+                    size_t lda, const float *b, size_t ldb, float beta, float *c, size_t ldc,
+                    bool trans_b, bool lower_only, size_t lds_pad) {
+    return gemm_impl<float>(s, "gemm_f32", m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, trans_b,
+                            lower_only, lds_pad);
 }
 
 int gemm_lowp(hipStream_t s, int prec, size_t m, size_t n, size_t k, float alpha,

@@ -52,6 +52,25 @@ inline int gemm_t<float>(hipStream_t s, size_t m, size_t n, size_t k,
     return gemm_f32(s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tb, lo);
 }
 
+// As gemm_t, with lds_pad bytes of dynamic LDS per workgroup that the kernel
+// does not use: fewer of its workgroups then share a CU, which leaves room
+// beside it for another kernel's workgroups (the general kernel only).
+int gemm_f64_padded(hipStream_t s, size_t m, size_t n, size_t k, double alpha, const double *a,
+                    size_t lda, const double *b, size_t ldb, double beta, double *c, size_t ldc,
+                    bool trans_b, bool lower_only, size_t lds_pad);
+int gemm_f32_padded(hipStream_t s, size_t m, size_t n, size_t k, float alpha, const float *a,
+                    size_t lda, const float *b, size_t ldb, float beta, float *c, size_t ldc,
+                    bool trans_b, bool lower_only, size_t lds_pad);
+template <class T>
+inline int gemm_t_padded(hipStream_t s, size_t m, size_t n, size_t k, T alpha, const T *a, size_t lda,
+                         const T *b, size_t ldb, T beta, T *c, size_t ldc, bool tb, bool lo,
+                         size_t lds_pad) {
+    if constexpr (sizeof(T) == 8)
+        return gemm_f64_padded(s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tb, lo, lds_pad);
+    else
+        return gemm_f32_padded(s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tb, lo, lds_pad);
+}
+
 // y = A x (FP64, one row per wave), A m x n.
 int gemv_f64(hipStream_t s, size_t m, size_t n, const double *a, size_t lda,
              const double *x, double *y);
