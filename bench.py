@@ -36,7 +36,7 @@ PEAKS = {"f64": 78.6, "f32": 157.3, "f16": 2500.0, "bf16": 2500.0,
          "f16x3": 2500.0 / 3}  # dense MFMA TFLOP/s
 HBM_PEAK = 8000.0  # GB/s
 PREC = {"f32": 1, "f16": 2, "bf16": 3, "f16x3": 4}
-PIVOT = {"partial": 0, "tournament": 1}
+PIVOT = {"partial": 0, "tournament": 1, "auto": 2}
 
 
 def parse():
@@ -48,7 +48,7 @@ def parse():
     p.add_argument("--nrhs", type=int, default=1)
     p.add_argument("--kappa", type=float, default=1e4)
     p.add_argument("--prec", default="f16x3", choices=sorted(PREC))
-    p.add_argument("--pivot", default="partial", choices=sorted(PIVOT),
+    p.add_argument("--pivot", default="auto", choices=sorted(PIVOT),
                    help="pivoting of the low-precision factorization")
     p.add_argument("--plain-steps", type=int, default=None)
     p.add_argument("--no-cpu", action="store_true")
@@ -310,7 +310,9 @@ def main():
         "dtype": (f"{args.prec} factor + f32 panels" if args.prec != "f32" else "f32 factor") + ", f64 refinement", "data": "synthetic",
         "config": {"workload": f"refined mixed-precision LU solve, A = U diag(s) V^T, n={n}, "
                                f"kappa={args.kappa:g}, {nrhs} RHS", "n": n, "nrhs": nrhs,
-                   "kappa": args.kappa, "factor_prec": args.prec, "pivot": args.pivot, "tol": 1e-13},
+                   "kappa": args.kappa, "factor_prec": args.prec,
+                   "pivot": args.pivot if args.pivot != "auto" else
+                   ("tournament" if args.prec in ("f32", "f16x3") else "partial"), "tol": 1e-13},
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
         "backward_error_max": eta_max, "ir_iterations": iters, "fallbacks": fell_back,

@@ -319,7 +319,14 @@ enum {
 };
 enum {
     RLG_PIVOT_PARTIAL = 0,     /* partial pivoting, as PartialPivLu (lu.rs:170-178) */
-    RLG_PIVOT_TOURNAMENT = 1   /* tournament pivoting (TSLU) on the FP32 panels */
+    RLG_PIVOT_TOURNAMENT = 1,  /* tournament pivoting (TSLU) on the FP32 panels: its
+                                  panels never wait for one another, so above order
+                                  1024 the trailing update overlaps them on a second
+                                  stream */
+    RLG_PIVOT_AUTO = 2         /* the default: tournament with FP32 or split-FP16
+                                  updates, partial with FP16 or BF16 updates (whose
+                                  factors, with tournament pivots, are too weak a
+                                  preconditioner at kappa 1e2-1e4) */
 };
 typedef struct rlg_mixed_opts {
     int factor_prec;      /* RLG_PREC_* */

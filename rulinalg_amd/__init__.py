@@ -581,7 +581,7 @@ class MixedOptions:
     """Options of the mixed-precision refined solve (include/rlgpu.h)."""
 
     def __init__(self, factor_prec=_lib.PREC_F16X3, refine=_lib.REFINE_IR, tol=1e-13, max_iter=30,
-                 fallback=True, pivot=_lib.PIVOT_PARTIAL):
+                 fallback=True, pivot=_lib.PIVOT_AUTO):
         self.factor_prec, self.refine, self.tol = factor_prec, refine, tol
         self.max_iter, self.fallback, self.pivot = max_iter, fallback, pivot
 

@@ -19,7 +19,7 @@ STATUS_NAMES = {
 
 PREC_F64, PREC_F32, PREC_F16, PREC_BF16, PREC_F16X3 = 0, 1, 2, 3, 4
 REFINE_IR, REFINE_GMRES = 0, 1
-PIVOT_PARTIAL, PIVOT_TOURNAMENT = 0, 1
+PIVOT_PARTIAL, PIVOT_TOURNAMENT, PIVOT_AUTO = 0, 1, 2
 KIND_LU, KIND_CHOL = 0, 1
 
 sz = C.c_size_t

@@ -178,20 +178,6 @@ size_t chol_work_bytes(size_t n, size_t elem) {
     return chol_div_bytes(n, elem) + ((ceil_div(n, CNB) * sizeof(unsigned) + 255) & ~(size_t)255);
 }
 
-// A second stream per host thread and device (default priority, created on
-// first use and kept): the look-ahead's trailing updates run there, so that
-// factorizations issued from different threads do not queue behind each
-// other's updates.
-static int chol_aux_stream(hipStream_t *out) {
-    static thread_local hipStream_t aux[64] = {};
-    int id = 0;
-    RLG_HIP(hipGetDevice(&id));
-    if (id < 0 || id >= 64) { set_error("device index %d out of range", id); return RLG_E_DEVICE; }
-    if (!aux[id]) RLG_HIP(hipStreamCreateWithFlags(&aux[id], hipStreamNonBlocking));
-    *out = aux[id];
-    return RLG_OK;
-}
-
 // LDS the look-ahead's update workgroups reserve beyond their own 66 KB
 // (FP64): two of them no longer fit on a CU's 160 KB, one of them and a
 // block-column workgroup (67 KB) do, so the block columns find a place on
@@ -227,7 +213,7 @@ static int chol_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info, v
     CholEvents ev;
     bool pending = false;  // an update of columns beyond the next block is running on s2
     if (native && n > 2 * (size_t)CONB) {
-        RLG_TRY(chol_aux_stream(&s2));
+        RLG_TRY(aux_stream(&s2));
         RLG_HIP(hipEventCreateWithFlags(&ev.ready, hipEventDisableTiming));
         RLG_HIP(hipEventCreateWithFlags(&ev.done, hipEventDisableTiming));
     }

@@ -50,8 +50,10 @@ struct Device {
 // created lazily and live until process exit.
 int current_device(Device **dev);
 hipStream_t pick_stream(Device *dev, void *user_stream);
-// A second stream of the current device (default priority, created once)
-// for work that overlaps the caller's stream, ordered by events.
+// A second stream of the current device for the calling host thread
+// (default priority, created on first use and kept), for work that overlaps
+// the caller's stream, ordered by events: factorizations issued from
+// different threads do not queue behind each other's overlapped work.
 int aux_stream(hipStream_t *out);
 
 // ------------------------------------------------------------------ memory

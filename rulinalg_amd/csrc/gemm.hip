@@ -858,7 +858,7 @@ size_t gemm_lowp_pack_bytes(size_t m, size_t n, size_t k) {
 
 static int gemm_split_packed(hipStream_t s, size_t m, size_t n, size_t k, float alpha, const float *a,
                              size_t lda, const float *b, size_t ldb, float beta, float *c, size_t ldc,
-                             bool trans_b, bool lower_only, void *pack) {
+                             bool trans_b, bool lower_only, void *pack, size_t lds_pad) {
     const size_t kp = (k + PKB - 1) / PKB * PKB;
     char *p = static_cast<char *>(pack);
     _Float16 *ah = reinterpret_cast<_Float16 *>(p); p += al256z(m * kp * 2);
@@ -884,13 +884,13 @@ static int gemm_split_packed(hipStream_t s, size_t m, size_t n, size_t k, float 
     const double flops = (lower_only ? 1.0 : 2.0) * (double)m * (double)n * (double)k;
     RLG_PROF_W("gemm_f16x3", s, flops);
     if (lower_only)
-        hipLaunchKernelGGL(k_gemm_nt_split<true>, dim3(tiles_m * tiles_n), dim3(GTHREADS), 0, s, (int)m,
-                           (int)n, (int)kp, alpha, ah, al, ia, bh, bl, ib, (long)kp, beta, c, (long)ldc,
-                           tiles_m, tiles_n);
+        hipLaunchKernelGGL(k_gemm_nt_split<true>, dim3(tiles_m * tiles_n), dim3(GTHREADS), (unsigned)lds_pad,
+                           s, (int)m, (int)n, (int)kp, alpha, ah, al, ia, bh, bl, ib, (long)kp, beta, c,
+                           (long)ldc, tiles_m, tiles_n);
     else
-        hipLaunchKernelGGL(k_gemm_nt_split<false>, dim3(tiles_m * tiles_n), dim3(GTHREADS), 0, s, (int)m,
-                           (int)n, (int)kp, alpha, ah, al, ia, bh, bl, ib, (long)kp, beta, c, (long)ldc,
-                           tiles_m, tiles_n);
+        hipLaunchKernelGGL(k_gemm_nt_split<false>, dim3(tiles_m * tiles_n), dim3(GTHREADS), (unsigned)lds_pad,
+                           s, (int)m, (int)n, (int)kp, alpha, ah, al, ia, bh, bl, ib, (long)kp, beta, c,
+                           (long)ldc, tiles_m, tiles_n);
     RLG_LAUNCHED();
     return RLG_OK;
 }
@@ -1018,7 +1018,8 @@ int gemm_f32_padded(hipStream_t s, size_t m, size_t n, size_t k, float alpha, co
 
 int gemm_lowp(hipStream_t s, int prec, size_t m, size_t n, size_t k, float alpha,
               const float *a, size_t lda, const float *b, size_t ldb, float beta, float *c,
-              size_t ldc, bool trans_b, bool lower_only, void *pack, size_t pack_bytes) {
+              size_t ldc, bool trans_b, bool lower_only, void *pack, size_t pack_bytes,
+              size_t lds_pad) {
     if (m == 0 || n == 0) return RLG_OK;
     RLG_TRY(check_dims(m, n, k));
     if (prec == RLG_PREC_F16X3) {
@@ -1026,7 +1027,8 @@ int gemm_lowp(hipStream_t s, int prec, size_t m, size_t n, size_t k, float alpha
             set_error("gemm_lowp: the split FP16 form needs its packed-operand area");
             return RLG_E_INVALID_ARG;
         }
-        return gemm_split_packed(s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, trans_b, lower_only, pack);
+        return gemm_split_packed(s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, trans_b, lower_only, pack,
+                                 lds_pad);
     }
     if (k > 0 && pack && pack_bytes >= gemm_lowp_pack_bytes(m, n, k)) {
         if (prec == RLG_PREC_F16)

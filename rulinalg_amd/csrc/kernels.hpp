@@ -30,7 +30,7 @@ int gemm_lowp(hipStream_t s, int prec, size_t m, size_t n, size_t k,
               float alpha, const float *a, size_t lda, const float *b,
               size_t ldb, float beta, float *c, size_t ldc,
               bool trans_b = false, bool lower_only = false,
-              void *pack = nullptr, size_t pack_bytes = 0);
+              void *pack = nullptr, size_t pack_bytes = 0, size_t lds_pad = 0);
 size_t gemm_lowp_pack_bytes(size_t m, size_t n, size_t k);
 
 template <class T>

@@ -891,14 +891,32 @@ static int update_native(hipStream_t s, size_t m, size_t n, size_t k, const T *a
     return gemm_t<T>(s, m, n, k, T(-1), a, lda, b, ldb, T(1), c, ldc);
 }
 
-// set by lu_factor_lowp for its duration: precision and the packed-operand area
+// set by lu_factor_lowp for its duration: precision and the packed-operand
+// areas (one for the caller's stream, one for the look-ahead's second stream)
 static thread_local int g_lowp_prec = RLG_PREC_F16;
 static thread_local void *g_lowp_pack = nullptr;
+static thread_local void *g_lowp_pack2 = nullptr;
 static thread_local size_t g_lowp_pack_bytes = 0;
 static int update_lowp(hipStream_t s, size_t m, size_t n, size_t k, const float *a, size_t lda,
                        const float *b, size_t ldb, float *c, size_t ldc) {
     return gemm_lowp(s, g_lowp_prec, m, n, k, -1.f, a, lda, b, ldb, 1.f, c, ldc, false, false,
                      g_lowp_pack, g_lowp_pack_bytes);
+}
+
+// The look-ahead's update on the second stream.  Its workgroups reserve LDS
+// they do not use, so that one of them and one workgroup of the panel chain
+// (tournament, row swaps, block-row solve, depth-64 update: at most 67 KB)
+// share a CU, where two of them would fill its 160 KB: the split FP16
+// kernel stages 80 KB, the FP32 one 66 KB.
+constexpr size_t LU_SPLIT_PAD = 8 << 10, LU_F32_PAD = 24 << 10;
+static int update_lowp_aux(hipStream_t s, size_t m, size_t n, size_t k, const float *a, size_t lda,
+                           const float *b, size_t ldb, float *c, size_t ldc) {
+    return gemm_lowp(s, g_lowp_prec, m, n, k, -1.f, a, lda, b, ldb, 1.f, c, ldc, false, false,
+                     g_lowp_pack2, g_lowp_pack_bytes, LU_SPLIT_PAD);
+}
+static int update_f32_aux(hipStream_t s, size_t m, size_t n, size_t k, const float *a, size_t lda,
+                          const float *b, size_t ldb, float *c, size_t ldc) {
+    return gemm_t_padded<float>(s, m, n, k, -1.f, a, lda, b, ldb, 1.f, c, ldc, false, false, LU_F32_PAD);
 }
 
 // Events used to order the look-ahead stream against the main stream.
@@ -934,20 +952,27 @@ static int outer_panel(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv,
     return RLG_OK;
 }
 
-// Right-looking blocked LU with look-ahead.  After outer block k is factored
-// and its block row solved, the trailing update is split: the columns of
-// block k+1 are updated first on the main stream, which then factors block
-// k+1 while the rest of the trailing update runs on the auxiliary stream.
-// The next step's row swaps and block-row solve wait for that update.
+// Right-looking blocked LU, with look-ahead when `aux_update` is given.
+// After outer block k is factored and its block row solved, the trailing
+// update is split: the columns of block k+1 are updated first on the
+// caller's stream, which then factors block k+1, while the rest of the
+// trailing update runs on a second stream (`aux_update`, one stream per host
+// thread and device, default priority).  The next step's row swaps and
+// block-row solve wait for that update on an event.  The two streams touch
+// disjoint columns meanwhile (block k+1's own panels, swaps and updates stay
+// inside its columns), every entry receives the same updates in the same
+// order, and the split follows the update's 128-wide tile grid, so the
+// factors are the same as without look-ahead.
 //
-// LOOK_AHEAD is off: a run with it on stalled on the GPU and the stall has
-// not been traced to its cause, so every update runs on the main stream.
-constexpr bool LOOK_AHEAD = false;
-// The packed low-precision operand area serves one trailing update at a time.
-static_assert(!LOOK_AHEAD, "concurrent trailing updates would share the packed operand area");
+// Look-ahead is used with the tournament panels only.  Their workgroups
+// never wait for one another, so they run beside the update on whatever
+// CUs are free.  The partial-pivoting panel is a cooperative launch, which
+// the runtime does not overlap with other streams' kernels.  An earlier form
+// with a non-cooperative exchange panel beside the second stream stalled on
+// the GPU.
 template <class T>
 static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv, int64_t *info,
-                          void *work, UpdateFn<T> big_update) {
+                          void *work, UpdateFn<T> big_update, UpdateFn<T> aux_update) {
     LuWork lw = carve(work, n, sizeof(T));
     // exchange tags restart with every factorization
     RLG_TRY(fill_u32(reinterpret_cast<uint32_t *>(lw.xch), 2 * xch_words(sizeof(T)), 0u, s));
@@ -956,7 +981,7 @@ static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ip
     if (n == 0) return RLG_OK;
     hipStream_t s2 = nullptr;
     StepEvents ev;
-    if (LOOK_AHEAD && n > 2 * (size_t)ONB) {
+    if (aux_update && n > 2 * (size_t)ONB) {
         RLG_TRY(aux_stream(&s2));
         RLG_HIP(hipEventCreateWithFlags(&ev.ready, hipEventDisableTiming));
         RLG_HIP(hipEventCreateWithFlags(&ev.big, hipEventDisableTiming));
@@ -984,12 +1009,12 @@ static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ip
             RLG_TRY(big_update(s, n - c0, n - c0, kb, a + c0 * lda + k0, lda, a + k0 * lda + c0, lda,
                                a + c0 * lda + c0, lda));
         } else {
-            // look-ahead columns first, then the rest on the auxiliary stream
+            // look-ahead columns first, then the rest on the second stream
             RLG_TRY(big_update(s, n - c0, kb2, kb, a + c0 * lda + k0, lda, a + k0 * lda + c0, lda,
                                a + c0 * lda + c0, lda));
             RLG_HIP(hipEventRecord(ev.ready, s));
             RLG_HIP(hipStreamWaitEvent(s2, ev.ready, 0));
-            RLG_TRY(big_update(s2, n - c0, n - c0 - kb2, kb, a + c0 * lda + k0, lda,
+            RLG_TRY(aux_update(s2, n - c0, n - c0 - kb2, kb, a + c0 * lda + k0, lda,
                                a + k0 * lda + c0 + kb2, lda, a + c0 * lda + c0 + kb2, lda));
             RLG_HIP(hipEventRecord(ev.big, s2));
             ev.big_pending = true;
@@ -1002,34 +1027,40 @@ static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ip
 
 template <class T>
 int lu_factor(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv, int64_t *info, void *work) {
-    return lu_factor_impl<T>(s, n, a, lda, ipiv, info, work, &update_native<T>);
+    return lu_factor_impl<T>(s, n, a, lda, ipiv, info, work, &update_native<T>, nullptr);
 }
 template int lu_factor<double>(hipStream_t, size_t, double *, size_t, int64_t *, int64_t *, void *);
 template int lu_factor<float>(hipStream_t, size_t, float *, size_t, int64_t *, int64_t *, void *);
 
-// The trailing updates are at most n x n with depth ONB.
+// The trailing updates are at most n x n with depth ONB; the look-ahead's
+// second stream has a packed-operand area of its own.
 size_t lu_lowp_work_bytes(size_t n) {
-    return align_up(lu_work_bytes(n, 4)) + align_up(gemm_lowp_pack_bytes(n, n, ONB)) +
+    return align_up(lu_work_bytes(n, 4)) + 2 * align_up(gemm_lowp_pack_bytes(n, n, ONB)) +
            tslu_work_bytes(n);
 }
 
 int lu_factor_lowp(hipStream_t s, int prec, int pivot, size_t n, float *a, size_t lda, int64_t *ipiv,
                    int64_t *info, void *work) {
     char *w = static_cast<char *>(work);
+    const size_t pack = align_up(gemm_lowp_pack_bytes(n, n, ONB));
     g_lowp_prec = prec;
     g_lowp_pack = w + align_up(lu_work_bytes(n, 4));
+    g_lowp_pack2 = static_cast<char *>(g_lowp_pack) + pack;
     g_lowp_pack_bytes = gemm_lowp_pack_bytes(n, n, ONB);
     g_tslu_work = nullptr;
     g_panel_exact = false;
     int st = RLG_OK;
+    UpdateFn<float> aux = nullptr;
     if (pivot == RLG_PIVOT_TOURNAMENT) {
-        g_tslu_work = w + align_up(lu_work_bytes(n, 4)) + align_up(g_lowp_pack_bytes);
+        g_tslu_work = static_cast<char *>(g_lowp_pack2) + pack;
         st = tslu_reset(s, g_tslu_work, n);
+        aux = prec == RLG_PREC_F32 ? &update_f32_aux : &update_lowp_aux;
     }
     if (st == RLG_OK)
         st = lu_factor_impl<float>(s, n, a, lda, ipiv, info, work,
-                                   prec == RLG_PREC_F32 ? &update_native<float> : &update_lowp);
+                                   prec == RLG_PREC_F32 ? &update_native<float> : &update_lowp, aux);
     g_lowp_pack = nullptr;
+    g_lowp_pack2 = nullptr;
     g_lowp_pack_bytes = 0;
     g_tslu_work = nullptr;
     g_panel_exact = true;

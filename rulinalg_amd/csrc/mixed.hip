@@ -725,7 +725,7 @@ void rlg_mixed_opts_default(rlg_mixed_opts *o) {
     o->tol = 1e-13;
     o->max_iter = 30;
     o->fallback = 1;
-    o->pivot = RLG_PIVOT_PARTIAL;
+    o->pivot = RLG_PIVOT_AUTO;
 }
 
 int rlg_solve_mixed_f64_dev(size_t n, const double *a, size_t lda, size_t nrhs, const double *b,
@@ -738,10 +738,13 @@ int rlg_solve_mixed_f64_dev(size_t n, const double *a, size_t lda, size_t nrhs, 
         set_error("unknown factor precision %d", o.factor_prec);
         return RLG_E_INVALID_ARG;
     }
-    if (o.pivot != RLG_PIVOT_PARTIAL && o.pivot != RLG_PIVOT_TOURNAMENT) {
+    if (o.pivot != RLG_PIVOT_PARTIAL && o.pivot != RLG_PIVOT_TOURNAMENT && o.pivot != RLG_PIVOT_AUTO) {
         set_error("unknown pivoting %d", o.pivot);
         return RLG_E_INVALID_ARG;
     }
+    if (o.pivot == RLG_PIVOT_AUTO)  // tournament where the trailing updates keep FP32 accuracy
+        o.pivot = (o.factor_prec == RLG_PREC_F32 || o.factor_prec == RLG_PREC_F16X3) ? RLG_PIVOT_TOURNAMENT
+                                                                                       : RLG_PIVOT_PARTIAL;
     Device *d;
     RLG_TRY(current_device(&d));
     rlg_mixed_report r;

@@ -64,10 +64,9 @@ hipStream_t pick_stream(Device *dev, void *user_stream) {
 }
 
 int aux_stream(hipStream_t *out) {
+    static thread_local hipStream_t aux[64] = {nullptr};
     int id = 0;
     RLG_HIP(hipGetDevice(&id));
-    static hipStream_t aux[64] = {nullptr};
-    std::lock_guard<std::mutex> lk(g_dev_mu);
     if (id < 0 || id >= 64) { set_error("device index %d out of range", id); return RLG_E_DEVICE; }
     if (!aux[id]) RLG_HIP(hipStreamCreateWithFlags(&aux[id], hipStreamNonBlocking));
     *out = aux[id];

@@ -567,17 +567,19 @@ def test_mixed_refined_well_conditioned(prec):
         assert eta(a, x[i], b[i]) <= 1e-13
 
 
+@pytest.mark.parametrize("pivot", [_lib.PIVOT_TOURNAMENT, _lib.PIVOT_PARTIAL], ids=["tournament", "partial"])
 @pytest.mark.parametrize("prec,n", [(_lib.PREC_F32, 200), (_lib.PREC_F32, 1500), (_lib.PREC_F32, 4100),
                                     (_lib.PREC_F16, 200), (_lib.PREC_F16, 1500),
                                     (_lib.PREC_F16X3, 1500), (_lib.PREC_F16X3, 4100)])
-def test_mixed_tournament_pivoting(prec, n):
+def test_mixed_pivoting_modes(prec, n, pivot):
     """TSLU panels (256-row leaves merged four at a time: n = 1500 is a
     three-level tree, n = 4100 a four-level one whose last panel is 4 columns
-    wide): refinement must converge without the FP64 fallback, to the same
-    bar as partial pivoting."""
+    wide; above order 1024 the trailing update also overlaps the panels on a
+    second stream) and the exchange panel of partial pivoting: refinement
+    must converge without the FP64 fallback in both modes."""
     a = wl.conditioned(n, 1e3, 5, 6)
     b = wl.gaussian(2, n, 7)
-    x, rep = rl.solve_mixed(a, b, rl.MixedOptions(factor_prec=prec, pivot=_lib.PIVOT_TOURNAMENT))
+    x, rep = rl.solve_mixed(a, b, rl.MixedOptions(factor_prec=prec, pivot=pivot))
     assert rep["converged"] == 1 and rep["fell_back"] == 0
     for i in range(2):
         assert eta(a, x[i], b[i]) <= 1e-13
