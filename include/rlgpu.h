@@ -320,9 +320,9 @@ enum {
 enum {
     RLG_PIVOT_PARTIAL = 0,     /* partial pivoting, as PartialPivLu (lu.rs:170-178) */
     RLG_PIVOT_TOURNAMENT = 1,  /* tournament pivoting (TSLU) on the FP32 panels: its
-                                  panels never wait for one another, so above order
-                                  1024 the trailing update overlaps them on a second
-                                  stream */
+                                  panels never wait for one another, so from order
+                                  8192 on the trailing update overlaps them on a
+                                  second stream */
     RLG_PIVOT_AUTO = 2         /* the default: tournament with FP32 or split-FP16
                                   updates, partial with FP16 or BF16 updates (whose
                                   factors, with tournament pivots, are too weak a

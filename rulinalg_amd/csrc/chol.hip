@@ -212,7 +212,9 @@ static int chol_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info, v
     hipStream_t s2 = nullptr;
     CholEvents ev;
     bool pending = false;  // an update of columns beyond the next block is running on s2
-    if (native && n > 2 * (size_t)CONB) {
+    // from order 8192 on: below it factorizations mostly run several at a
+    // time (the serving loop), and those already share the device
+    if (native && n >= 8192) {
         RLG_TRY(aux_stream(&s2));
         RLG_HIP(hipEventCreateWithFlags(&ev.ready, hipEventDisableTiming));
         RLG_HIP(hipEventCreateWithFlags(&ev.done, hipEventDisableTiming));

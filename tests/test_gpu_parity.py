@@ -574,8 +574,7 @@ def test_mixed_refined_well_conditioned(prec):
 def test_mixed_pivoting_modes(prec, n, pivot):
     """TSLU panels (256-row leaves merged four at a time: n = 1500 is a
     three-level tree, n = 4100 a four-level one whose last panel is 4 columns
-    wide; above order 1024 the trailing update also overlaps the panels on a
-    second stream) and the exchange panel of partial pivoting: refinement
+    wide) and the exchange panel of partial pivoting: refinement
     must converge without the FP64 fallback in both modes."""
     a = wl.conditioned(n, 1e3, 5, 6)
     b = wl.gaussian(2, n, 7)
