@@ -209,13 +209,15 @@ static int chol_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *info, v
     RLG_TRY(fill_i64(info, 1, (int64_t)n, s));
     RLG_TRY(fill_u32(arrive, ceil_div(n, CNB), 0u, s));
     const bool native = lowp_prec != RLG_PREC_F16 && lowp_prec != RLG_PREC_BF16;
+    AuxStream aux;
     hipStream_t s2 = nullptr;
     CholEvents ev;
     bool pending = false;  // an update of columns beyond the next block is running on s2
     // from order 8192 on: below it factorizations mostly run several at a
     // time (the serving loop), and those already share the device
     if (native && n >= 8192) {
-        RLG_TRY(aux_stream(&s2));
+        RLG_TRY(aux.create());
+        s2 = aux.s;
         RLG_HIP(hipEventCreateWithFlags(&ev.ready, hipEventDisableTiming));
         RLG_HIP(hipEventCreateWithFlags(&ev.done, hipEventDisableTiming));
     }

@@ -50,11 +50,19 @@ struct Device {
 // created lazily and live until process exit.
 int current_device(Device **dev);
 hipStream_t pick_stream(Device *dev, void *user_stream);
-// A second stream of the current device for the calling host thread
-// (default priority, created on first use and kept), for work that overlaps
-// the caller's stream, ordered by events: factorizations issued from
-// different threads do not queue behind each other's overlapped work.
-int aux_stream(hipStream_t *out);
+// A second stream of the current device (default priority) for work that
+// overlaps the caller's stream, ordered by events.  It lives for one call:
+// the runtime's cooperative launches on the process's other streams measured
+// markedly slower while extra streams existed (the FP64 LU at order 16384,
+// whose panels are cooperative: 142 -> 196 ms with two kept alive).
+struct AuxStream {
+    hipStream_t s = nullptr;
+    int create();
+    ~AuxStream();
+    AuxStream() = default;
+    AuxStream(const AuxStream &) = delete;
+    AuxStream &operator=(const AuxStream &) = delete;
+};
 
 // ------------------------------------------------------------------ memory
 

@@ -957,8 +957,8 @@ static int outer_panel(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ipiv,
 // After outer block k is factored and its block row solved, the trailing
 // update is split: the columns of block k+1 are updated first on the
 // caller's stream, which then factors block k+1, while the rest of the
-// trailing update runs on a second stream (`aux_update`, one stream per host
-// thread and device, default priority).  The next step's row swaps and
+// trailing update runs on a second stream (`aux_update`; a stream created
+// for the call, default priority).  The next step's row swaps and
 // block-row solve wait for that update on an event.  The two streams touch
 // disjoint columns meanwhile (block k+1's own panels, swaps and updates stay
 // inside its columns), every entry receives the same updates in the same
@@ -984,10 +984,12 @@ static int lu_factor_impl(hipStream_t s, size_t n, T *a, size_t lda, int64_t *ip
     RLG_TRY(fill_u32(reinterpret_cast<uint32_t *>(lw.timeout), 1, 0u, s));
     RLG_TRY(fill_i64(info, 1, (int64_t)n, s));
     if (n == 0) return RLG_OK;
+    AuxStream aux;
     hipStream_t s2 = nullptr;
     StepEvents ev;
     if (aux_update && n >= LOOKAHEAD_MIN_N) {
-        RLG_TRY(aux_stream(&s2));
+        RLG_TRY(aux.create());
+        s2 = aux.s;
         RLG_HIP(hipEventCreateWithFlags(&ev.ready, hipEventDisableTiming));
         RLG_HIP(hipEventCreateWithFlags(&ev.big, hipEventDisableTiming));
     }

@@ -63,14 +63,13 @@ hipStream_t pick_stream(Device *dev, void *user_stream) {
     return user_stream ? static_cast<hipStream_t>(user_stream) : dev->stream;
 }
 
-int aux_stream(hipStream_t *out) {
-    static thread_local hipStream_t aux[64] = {nullptr};
-    int id = 0;
-    RLG_HIP(hipGetDevice(&id));
-    if (id < 0 || id >= 64) { set_error("device index %d out of range", id); return RLG_E_DEVICE; }
-    if (!aux[id]) RLG_HIP(hipStreamCreateWithFlags(&aux[id], hipStreamNonBlocking));
-    *out = aux[id];
+int AuxStream::create() {
+    RLG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     return RLG_OK;
+}
+
+AuxStream::~AuxStream() {
+    if (s) (void)hipStreamDestroy(s);
 }
 
 // ------------------------------------------------------------------ profiler
