@@ -51,10 +51,12 @@ struct Device {
 int current_device(Device **dev);
 hipStream_t pick_stream(Device *dev, void *user_stream);
 // A second stream of the current device (default priority) for work that
-// overlaps the caller's stream, ordered by events.  It lives for one call:
-// the runtime's cooperative launches on the process's other streams measured
-// markedly slower while extra streams existed (the FP64 LU at order 16384,
-// whose panels are cooperative: 142 -> 196 ms with two kept alive).
+// overlaps the caller's stream, ordered by events.  It lives for one call.
+// With two such streams kept per thread after the mixed LU's look-ahead had
+// used them, the FP64 LU at order 16384 (cooperative panels) measured
+// 196 ms instead of 142 ms; created per call, it does not.  Other streams,
+// idle or after running a kernel, did not slow it (measured at order 4096),
+// so the cause lies in how the kept streams had been used.
 struct AuxStream {
     hipStream_t s = nullptr;
     int create();
