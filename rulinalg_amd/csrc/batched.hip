@@ -1,10 +1,10 @@
 // Batched factor-and-solve of many independent systems of uneven order in
 // one persistent launch.
 //
-// Grid: as many workgroups as can be resident (2 per CU); each workgroup
-// takes systems from a device work counter (longest first, the order the
-// host computes) until the list is exhausted, so every workgroup reaches the
-// exit test and the grid drains.  One workgroup factors and solves one
+// Grid: one workgroup per CU (the kernel's registers allow no second); each
+// workgroup takes systems from a device work counter (longest first, the
+// order the host computes) until the list is exhausted, so every workgroup
+// reaches the exit test and the grid drains.  One workgroup factors and solves one
 // system entirely:
 //
 //   LU (PartialPivLu::decompose + solve, lu.rs:163-244): right-looking,
