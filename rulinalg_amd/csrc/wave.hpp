@@ -33,7 +33,7 @@ __device__ __forceinline__ double lane_get(double x, int l) {
     const long long b = __double_as_longlong(x);
     const int lo = __builtin_amdgcn_readlane((int)b, l);
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
 // DPP lane permutations within rows of 16 lanes.
@@ -51,7 +51,7 @@ template <int CTRL> __device__ __forceinline__ float dpp(float x) {
 template <int CTRL> __device__ __forceinline__ double dpp(double x) {
     const long long b = __double_as_longlong(x);
     const int lo = dpp<CTRL>((int)b), hi = dpp<CTRL>((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
 __device__ __forceinline__ unsigned long long lane_get(unsigned long long x, int l) {
