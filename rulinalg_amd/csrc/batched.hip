@@ -36,7 +36,7 @@ int nb_for(int n) {
 }
 
 // One workgroup per CU: the kernels take 256 VGPRs + 256 AGPRs per lane
-// (one wave per SIMD) and 71 KB of LDS, so a second workgroup would only
+// (one wave per SIMD) and 73 KB of LDS, so a second workgroup would only
 // wait for the first to finish.
 int resident_grid(Device *d) { return d->num_cus; }
 

@@ -77,7 +77,12 @@ struct Smem {
     int red_i[BTH / 64];
     int piv[32];
     double div[32];
-    double prow[32], jrow[32];  // pivot row and row j of a panel column step
+    // LU column step j (buffers by the parity of j): each wave's pivot
+    // candidate (|value|, row, the whole row) and row j of the panel
+    double cand_v[2][BTH / 64];
+    int cand_i[2][BTH / 64];
+    double cand[2][BTH / 64][32];
+    double jrow[2][32];
     int mdst[64], msrc[64];     // net row moves of a panel's swaps
     int mcnt;
     int info;
@@ -171,13 +176,31 @@ __device__ int lu_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
         const int kb = min(NB, n - k), m = n - k;
         panel_move<T, LD, false, false>(A, n, k, m, kb, P);
         __syncthreads();
-        // Column j of the panel, two barriers: (1) the waves' candidates are
-        // in LDS; every thread picks the same pivot row p and reads what it
-        // needs of rows j and p and of its own rows; (2) the rows are
-        // rewritten (swap, multipliers, update of the columns right of j) and
-        // each thread searches column j + 1 over its own rows, which only it
-        // writes.  Thread t owns the panel rows t + q BTH.
+        // Column j of the panel, one barrier: before it, each wave's best
+        // candidate for column j (|value|, row and the whole row) and row j
+        // are copied to buffers of the column's parity; after it every
+        // thread picks the same pivot row p from the four candidates, threads
+        // j.. write the pivot row into row j, and each thread eliminates its
+        // own rows, taking the pivot row and (for row p) row j from the
+        // copies, then searches column j + 1 over its rows and the waves
+        // publish the next candidates into the other parity's buffers.
+        // Thread t owns the panel rows t + q BTH and is the only one to write
+        // them, apart from row j.  Rows are read across threads only through
+        // the copies, which a wave makes from rows its own lanes wrote.
         constexpr int RQ = NB == 32 ? 1 : 2;  // NB = 32 only for m <= 247
+        T *cand = reinterpret_cast<T *>(&sm.cand[0][0][0]);  // [2][BNW][NB]
+        T *jrows = reinterpret_cast<T *>(&sm.jrow[0][0]);     // [2][NB]
+        // publish this wave's candidate (bv, bi: the lane's own) for the
+        // column of parity `par`, and (wave 0) row `jn` set aside
+        auto publish = [&](T bv, int bi, int par, int jn) __attribute__((always_inline)) {
+            const int wl = wave_first_max_lane(bv, bi);
+            bv = wl >= 0 ? lane_get(bv, wl) : T(0);
+            bi = wl >= 0 ? lane_get(bi, wl) : INT_MAX;
+            if (lane < NB)
+                cand[(par * BNW + wave) * NB + lane] = (bi != INT_MAX && lane < kb) ? P[bi * LD + lane] : T(0);
+            if (lane == 0) { sm.cand_v[par][wave] = (double)bv; sm.cand_i[par][wave] = bi; }
+            if (wave == 0 && lane < NB) jrows[par * NB + lane] = lane < kb ? P[jn * LD + lane] : T(0);
+        };
         {
             T bv = T(0);
             int bi = INT_MAX;
@@ -189,40 +212,31 @@ __device__ int lu_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
                 bv = take ? v : bv;
                 bi = take ? r : bi;
             }
-            const int wl = wave_first_max_lane(bv, bi);
-            bv = wl >= 0 ? lane_get(bv, wl) : T(0);
-            bi = wl >= 0 ? lane_get(bi, wl) : INT_MAX;
-            if (lane == 0) { sm.red_v[wave] = (double)bv; sm.red_i[wave] = bi; }
+            publish(bv, bi, 0, 0);
         }
         __syncthreads();
-        T *prow = reinterpret_cast<T *>(sm.prow), *jrow = reinterpret_cast<T *>(sm.jrow);
         for (int j = 0; j < kb; ++j) {
 #pragma clang fp contract(off)
-            T pa = (T)sm.red_v[0];
-            int p = sm.red_i[0];
+            const int par = j & 1;
+            T pa = (T)sm.cand_v[par][0];
+            int p = sm.cand_i[par][0], ww = 0;
 #pragma unroll
             for (int q = 1; q < BNW; ++q) {
-                const bool take = better((T)sm.red_v[q], sm.red_i[q], pa, p);
-                pa = take ? (T)sm.red_v[q] : pa;
-                p = take ? sm.red_i[q] : p;
+                const bool take = better((T)sm.cand_v[par][q], sm.cand_i[par][q], pa, p);
+                pa = take ? (T)sm.cand_v[par][q] : pa;
+                p = take ? sm.cand_i[par][q] : p;
+                ww = take ? q : ww;
             }
             p = __builtin_amdgcn_readfirstlane(p);  // the same in every lane
-            const T piv = P[p * LD + j];
-            // rows p and j set aside (the pivot row with zeros at columns <= j:
-            // the sweep below then leaves those columns as they are)
-            if (tid < NB) {
-                prow[tid] = (tid < kb) ? P[p * LD + tid] : T(0);
-                jrow[tid] = (tid < kb) ? P[j * LD + tid] : T(0);
-            }
+            ww = __builtin_amdgcn_readfirstlane(ww);
+            const T *prow = cand + (par * BNW + ww) * NB;  // row p (zeros at columns >= kb)
+            const T *jrow = jrows + par * NB;              // row j (zeros at columns >= kb)
+            const T piv = prow[j];
             if (tid == 0) {
                 sm.piv[j] = p;
                 if (pa < beps<T>() && sm.info == n) sm.info = k + j;
             }
-            __syncthreads();
-            if (tid < kb) {
-                P[j * LD + tid] = prow[tid];
-                if (tid < j) P[p * LD + tid] = jrow[tid];
-            }
+            if (tid < kb) P[j * LD + tid] = prow[tid];
             T bv = T(0);
             int bi = INT_MAX;
 #pragma unroll
@@ -248,12 +262,7 @@ __device__ int lu_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
                     }
                 }
             }
-            if (j + 1 < kb) {
-                const int wl = wave_first_max_lane(bv, bi);
-                bv = wl >= 0 ? lane_get(bv, wl) : T(0);
-                bi = wl >= 0 ? lane_get(bi, wl) : INT_MAX;
-                if (lane == 0) { sm.red_v[wave] = (double)bv; sm.red_i[wave] = bi; }
-            }
+            if (j + 1 < kb) publish(bv, bi, par ^ 1, j + 1);
             __syncthreads();
         }
         if (tid < kb) piv_out[k + tid] = k + sm.piv[tid];
