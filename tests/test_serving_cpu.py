@@ -165,12 +165,17 @@ def test_resident_reuse_and_lru_eviction():
 
 
 def test_mixed_mode_routes_large_lu_to_the_mixed_solve():
-    lib = StubLib()
-    loop = serving.ServingLoop(lib, torch_cpu(), "cpu", mixed=True)
-    reqs = [req(0, serving.KIND_LU, 600), req(1, serving.KIND_CHOL, 600), req(2, serving.KIND_LU, 64)]
-    loop.run(reqs)
-    assert [r.path for r in reqs] == ["single_mixed", "single", "batched_mixed"]
-    assert ("mixed", 600) in lib.calls
+    """Mixed mode: one-shot LU above order 512 takes the mixed solve,
+    Cholesky the FP64 path, and small systems the FP64 batched launch unless
+    `small_mixed` asks for the mixed batched one."""
+    for small_mixed, small_path in ((False, "batched"), (True, "batched_mixed")):
+        lib = StubLib()
+        loop = serving.ServingLoop(lib, torch_cpu(), "cpu", mixed=True, small_mixed=small_mixed)
+        reqs = [req(0, serving.KIND_LU, 600), req(1, serving.KIND_CHOL, 600), req(2, serving.KIND_LU, 64)]
+        loop.run(reqs)
+        assert [r.path for r in reqs] == ["single_mixed", "single", small_path]
+        assert ("mixed", 600) in lib.calls
+        loop.close()
 
 
 def test_replaced_factor_outlives_its_solves():
