@@ -159,7 +159,11 @@ def test_admission_budget(env):
     loop.close()
 
 
-def test_mixed_mode(env):
+@pytest.mark.parametrize("small_mixed", [False, True], ids=["small_fp64", "small_mixed"])
+def test_mixed_mode(env, small_mixed):
+    """Mixed mode: the one-shot LU above order 512 takes the mixed solve; the
+    small systems take the FP64 batched launch, or the mixed batched launch
+    with `small_mixed`; every result meets the FP64 backward-error bound."""
     lib, torch, dev = env
     r = wl.rng(89)
     specs = [(serving.KIND_LU, wl.conditioned(600, 1e3, 5, 6), r.standard_normal((1, 600)))]
@@ -168,11 +172,13 @@ def test_mixed_mode(env):
         specs.append((serving.KIND_LU, a, b))
     a, b = system(r, serving.KIND_CHOL, 150, 1)
     specs.append((serving.KIND_CHOL, a, b))
-    loop = serving.ServingLoop(lib, torch, dev, mixed=True)
+    loop = serving.ServingLoop(lib, torch, dev, mixed=True, small_mixed=small_mixed)
     reqs = [request(torch, dev, i, k, a, b) for i, (k, a, b) in enumerate(specs)]
     loop.run(reqs)
     torch.cuda.synchronize()
     assert reqs[0].path == "single_mixed"
+    small = "batched_mixed" if small_mixed else "batched"
+    assert [q.path for q in reqs if q.n <= 512] == [small] * 4
     for (kind, a, b), q in zip(specs, reqs):
         assert q.status == 0 and q.eta <= 1e-13
         x = q.x.cpu().numpy().reshape(b.shape)
