@@ -180,13 +180,15 @@ __device__ int lu_fused(T *__restrict__ A, int n, T *__restrict__ B, int nrhs,
         // candidate for column j (|value|, row and the whole row) and row j
         // are copied to buffers of the column's parity; after it every
         // thread picks the same pivot row p from the four candidates, threads
-        // j.. write the pivot row into row j, and each thread eliminates its
-        // own rows, taking the pivot row and (for row p) row j from the
+        // 0..kb-1 write the pivot row into row j, and each thread eliminates
+        // its own rows, taking the pivot row and (for row p) row j from the
         // copies, then searches column j + 1 over its rows and the waves
         // publish the next candidates into the other parity's buffers.
         // Thread t owns the panel rows t + q BTH and is the only one to write
         // them, apart from row j.  Rows are read across threads only through
-        // the copies, which a wave makes from rows its own lanes wrote.
+        // the copies, which a wave makes from rows its own lanes wrote (for
+        // column 0, from the loaded panel, after the barrier that follows
+        // the load).
         constexpr int RQ = NB == 32 ? 1 : 2;  // NB = 32 only for m <= 247
         T *cand = reinterpret_cast<T *>(&sm.cand[0][0][0]);  // [2][BNW][NB]
         T *jrows = reinterpret_cast<T *>(&sm.jrow[0][0]);     // [2][NB]
