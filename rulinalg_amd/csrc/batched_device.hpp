@@ -25,6 +25,7 @@
 // NB (8, 16 or 32) is chosen per system from an autotuned table of n-buckets
 // (rlg_batch_autotune); (n + 16) * (NB + 1) doubles must fit the panel
 // buffer.  The mixed form factors an FP32 copy in a per-workgroup workspace
+// and refines in FP64 against the untouched FP64 matrix.
 //
 // This header holds the device code only: the kernels and the phases they
 // call.  It depends on wave.hpp, on the item type and status codes of
