@@ -243,6 +243,31 @@ def test_mixed_fallback_and_failure_paths():
     assert st == [DIV_BY_ZERO, DECOMP_FAILURE]
 
 
+def test_random_mix_in_one_launch():
+    """A seeded random mix in one launch of each form: orders 1-512, 1-16
+    right-hand sides, both kinds, every panel width the host allows for the
+    order; each item against the oracle (plain) or to backward error 1e-13
+    with A left unmodified (mixed)."""
+    r = wl.rng(1500)
+    systems = []
+    for i in range(24):
+        n = int(r.integers(1, 513))
+        kind = LU if r.random() < 0.5 else CHOL
+        nb = int(r.choice(widths(n)))
+        a, b = system(r, kind, n, int(r.integers(1, 17)))
+        systems.append((kind, a, b, nb))
+    st, fb, out = run(systems, order=2, seed=5)
+    assert st == [OK] * len(systems)
+    for (kind, a, b, nb), (fa, x, ip) in zip(systems, out):
+        check(kind, a, b, fa, x, ip)
+    st, fb, out = run(systems, mode=1, grid=8, block=7)
+    assert st == [OK] * len(systems) and fb == 0
+    for (kind, a, b, nb), (fa, x, ip) in zip(systems, out):
+        assert np.array_equal(fa, a)
+        for i in range(b.shape[0]):
+            assert eta(a, x[i], b[i]) <= 1e-13
+
+
 @pytest.mark.parametrize("case, expect", [("clean", 0), ("barrier", 2), ("halfwave", 2), ("oob", 1)])
 def test_emulation_reports_planted_faults(case, expect):
     """Planted in small kernels of tests/emu/emu_selftest.cpp: a barrier that
